@@ -1,0 +1,143 @@
+/*
+ * tgtp_hip.h -- C ABI of the MI355X (gfx950) TGT+/SABA hot path.
+ *
+ * Plain C: fixed-width integers, plain pointers and sizes, int status codes,
+ * no exceptions across the boundary, caller-owned outputs.  One context per
+ * host thread; calls on one context are serialized by the caller.
+ *
+ * Reference interfaces replaced (ggokturk/tgtp):
+ *   tgtp_score          <- float* AESC::tgtp()            src/aesc.h:120-127
+ *                          (AESC(g, config) ctor           src/aesc.h:113-119)
+ *   tgtp_saba_walks     <- saba_randomwalk(src, len, n_walk, pvec, g)
+ *                          + its driver loop and reduction src/rw_experiment_saba.cpp:65-145, 190-207
+ *   tgtp_taus           <- AESC::calculate_taus(eps)      src/aesc.h:164-216
+ *   tgtp_graph_*        <- graph_t CSR (idx, adj, n, m)   src/graph.h:10-28
+ *   tgtp_load_txt       <- graph_t::load_txt               src/graph.h:29-97   (tgtp_graph_io.h)
+ *   tgtp_load_eigens_txt<- EigenVecs<float>::load_txt      src/aesc.h:51-64    (tgtp_graph_io.h)
+ *
+ * Data layout (host and device): idx uint32[n+1], adj uint32[m], rows sorted
+ * ascending, both directions of every undirected edge, duplicates kept
+ * (graph.h:13-15, 63-66).  Eigenvectors float[n*omega] vertex-major
+ * (X[v*omega + i] = component v of eigenvector i, aesc.h:60), eigenvalues
+ * double[omega] sorted by descending magnitude.
+ */
+#ifndef TGTP_HIP_H
+#define TGTP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------
+ * The reference reports invalid input by throwing std::runtime_error /
+ * std::invalid_argument at load time (graph.h:34,55,69,74; aesc.h:53-61,
+ * 87-89; main.cpp:27-30).  Across this ABI the same conditions return a
+ * negative code and leave a message in tgtp_ctx_last_error(). */
+#define TGTP_OK 0
+#define TGTP_E_ARG (-1)         /* invalid argument / input (runtime_error, invalid_argument) */
+#define TGTP_E_HIP (-2)         /* HIP runtime error */
+#define TGTP_E_NOMEM (-3)       /* host or device allocation failed (bad_alloc) */
+#define TGTP_E_IO (-4)          /* file could not be opened / parsed */
+#define TGTP_E_UNSUPPORTED (-5) /* shape outside what this build's kernels handle */
+
+typedef struct tgtp_ctx tgtp_ctx;       /* owns a HIP stream, events, scratch */
+typedef struct tgtp_dgraph tgtp_dgraph; /* device-resident CSR (HBM) */
+
+/* Host CSR view (the fields of fuser::graph_t, graph.h:13-15). */
+typedef struct tgtp_graph {
+  uint64_t n;          /* vertices */
+  uint64_t m;          /* CSR entries (both directions) */
+  const uint32_t* idx; /* [n+1] */
+  const uint32_t* adj; /* [m]   */
+} tgtp_graph;
+
+/* Eigenpairs (EigenVecs<float>, aesc.h:43-81). */
+typedef struct tgtp_eigens {
+  int32_t omega;          /* number of pairs, >= 3 */
+  const double* values;   /* [omega] */
+  const float* vectors;   /* [n*omega] vertex-major */
+} tgtp_eigens;
+
+/* Arithmetic of tgtp_score.
+ *  FAITHFUL  : every step in the reference's own float/double arithmetic and
+ *              order; scores equal the CPU restatement bit for bit.
+ *  FAST_FP32 : tau projection as a batched contraction with FP32 operands and
+ *              FP32 accumulation; per-edge score sums in FP64.
+ *  FAST_BF16 : as FAST_FP32 with BF16 eigenvector operands. */
+enum {
+  TGTP_MODE_FAITHFUL = 0,
+  TGTP_MODE_FAST_FP32 = 1,
+  TGTP_MODE_FAST_BF16 = 2
+};
+
+/* AESCConfig (aesc.h:98-102); the reference CLI sets delta = 1/n
+ * (main.cpp:36).  gamma >= 1, 0 < epsilon < 1. */
+typedef struct tgtp_params {
+  double epsilon;
+  double delta;
+  int32_t gamma;
+  int32_t mode; /* TGTP_MODE_* */
+} tgtp_params;
+
+/* Timings of the last call, from HIP events on the context's stream. */
+typedef struct tgtp_stats {
+  double kernel_ms;      /* device time of the call's kernels (event to event) */
+  double e2e_ms;         /* host wall time including H2D / D2H copies */
+  double walk_ms;        /* device time of the walk kernel(s) alone */
+  double taus_ms;        /* tgtp: tau stage */
+  double diffuse_ms;     /* tgtp: per-source diffusion + walk-count stage */
+  uint64_t transitions;  /* walks: n_sources_with_degree * n_walks * depth; tgtp: walker-hops */
+  uint64_t work_items;   /* tgtp: pushed edges (diffusion), walks: bouquets */
+  uint64_t host_fixups;  /* tgtp: results resolved on the host (documented cases) */
+} tgtp_stats;
+
+/* ---- context ------------------------------------------------------------ */
+int tgtp_ctx_create(int device, tgtp_ctx** out);
+void tgtp_ctx_destroy(tgtp_ctx* ctx);
+const char* tgtp_ctx_last_error(const tgtp_ctx* ctx);
+const char* tgtp_strerror(int code);
+
+/* ---- device-resident graph (inputs stay in HBM across calls) ------------- */
+int tgtp_graph_upload(tgtp_ctx* ctx, const tgtp_graph* g, tgtp_dgraph** out);
+void tgtp_graph_free(tgtp_ctx* ctx, tgtp_dgraph* dg);
+
+/* ---- SABA 1-way walks ------------------------------------------------------
+ * For every source s in [src_begin, src_end) with degree > 0, n_walks walks
+ * of exactly `depth` hops on the ordered uniform 32-bit grid, grouped into
+ * 8-lane bouquets, hashed steps with a whole-bouquet SplitMix32 fallback
+ * (rw_experiment_saba.cpp:65-145).  Counters are uint32 per vertex (the
+ * reference keeps per-thread float counters, rw_experiment_saba.cpp:192).
+ * counters_out [n] may be NULL; visit_count = sum(count), checksum =
+ * sum((v+1)*count[v]) in uint64 (rw_experiment_saba.cpp:203-206).
+ * n_walks must be a positive multiple of 8 (rw_experiment_saba.cpp:160-163).
+ * stats->kernel_ms covers counter clearing + walks + reduction, the window
+ * of t_rw_s (rw_experiment_saba.cpp:184-211) minus host work. */
+int tgtp_saba_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,
+                    uint32_t src_end, uint32_t depth, uint32_t n_walks,
+                    uint32_t* counters_out, uint64_t* visit_count,
+                    uint64_t* checksum, tgtp_stats* stats);
+
+/* ---- tau bounds (calculate_taus, aesc.h:164-216) ---------------------------
+ * eps_half is float(epsilon/2) as aesc.h:122 passes it.  taus_out [m]. */
+int tgtp_taus(tgtp_ctx* ctx, const tgtp_graph* g, const tgtp_eigens* e,
+              float eps_half, int32_t mode, int32_t* taus_out,
+              tgtp_stats* stats);
+
+/* ---- full TGT+ (AESC::tgtp, aesc.h:120-127) --------------------------------
+ * scores_out [m] float32 in CSR order: exactly the scores.bin payload
+ * (main.cpp:41).  Inputs are host arrays; they are copied to HBM. */
+int tgtp_score(tgtp_ctx* ctx, const tgtp_graph* g, const tgtp_eigens* e,
+               const tgtp_params* p, float* scores_out, tgtp_stats* stats);
+
+/* Same, on a graph already resident in HBM (eigenpairs are uploaded per call
+ * unless the same pointer was passed on the previous call with this dgraph). */
+int tgtp_score_dev(tgtp_ctx* ctx, const tgtp_dgraph* dg, const tgtp_eigens* e,
+                   const tgtp_params* p, float* scores_out, tgtp_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TGTP_HIP_H */

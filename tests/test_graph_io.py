@@ -1,0 +1,131 @@
+"""Host graph I/O (graph.h / aesc.h loader semantics) and the seeded
+generators.  CPU only."""
+import numpy as np
+import pytest
+
+import oracle
+from tgtp_amd import eigenprep as E
+from tgtp_amd import graph as G
+from tgtp_amd import workloads as W
+
+
+def _both(path):
+    g = G.load_txt(path)
+    oi, oa = oracle.load_txt(path)
+    assert np.array_equal(g.idx, oi) and np.array_equal(g.adj, oa)
+    return g
+
+
+def test_loader_comments_blanks_duplicates_selfloops(tmp_path):
+    p = tmp_path / "g.txt"
+    p.write_text("# header\n% mm comment\n\n   \n0 1\n1 2 extra fields ok\n  2 0\n"
+                 "0 1\n3 3\n\t# indented comment\n4\t0\r\n")
+    g = _both(p)
+    assert g.n == 5
+    # duplicates kept, self-loop stored twice (graph.h:63-66)
+    assert g.adj[g.idx[0]:g.idx[1]].tolist() == [1, 1, 2, 4]
+    assert g.adj[g.idx[3]:g.idx[4]].tolist() == [3, 3]
+
+
+def test_loader_max_id_defines_n_and_isolated_gap(tmp_path):
+    p = tmp_path / "g.txt"
+    p.write_text("0 1\n5 1\n")
+    g = _both(p)
+    assert g.n == 6 and g.degree(3) == 0
+    with pytest.raises(ValueError, match="isolated"):
+        G.check_aesc_graph(g, 2)
+    with pytest.raises(ValueError, match="omega"):
+        G.check_aesc_graph(g, 7)
+
+
+@pytest.mark.parametrize("text", ["0 -1\n", "0\n", "a b\n", "1.5 2\n", "0 4294967295\n",
+                                  "99999999999999999999 1\n"])
+def test_loader_rejects_invalid_edges(tmp_path, text):
+    p = tmp_path / "g.txt"
+    p.write_text("0 1\n" + text)
+    with pytest.raises(ValueError, match="invalid edge"):
+        G.load_txt(p)
+    with pytest.raises(ValueError, match="invalid edge"):
+        oracle.load_txt(p)
+
+
+def test_loader_empty_and_missing(tmp_path):
+    p = tmp_path / "g.txt"
+    p.write_text("# nothing\n")
+    with pytest.raises(ValueError, match="empty graph"):
+        G.load_txt(p)
+    with pytest.raises(OSError):
+        G.load_txt(tmp_path / "missing.txt")
+
+
+def test_eigen_loader_matches_oracle_and_rejects_nonfinite(tmp_path):
+    g = G.chung_lu(200, 2.5, 8.0, seed=4)
+    lam, X = E.eigens_f32(*E.compute_eigens(g, 12))
+    p = tmp_path / "e.txt"
+    E.write_eigens_txt(p, lam, X)
+    l1, X1 = G.load_eigens_txt(p, g.n, 12)
+    l2, X2 = oracle.load_eigens(p, g.n, 12)
+    assert np.array_equal(l1, lam) and np.array_equal(X1, X)
+    assert np.array_equal(l1, l2) and np.array_equal(X1, X2)
+    bad = tmp_path / "bad.txt"
+    bad.write_text(p.read_text().replace(p.read_text().split()[5], "nan", 1))
+    with pytest.raises(ValueError, match="invalid eigen"):
+        G.load_eigens_txt(bad, g.n, 12)
+
+
+def _simple_connected(g):
+    u = np.repeat(np.arange(g.n), g.degree())
+    assert not np.any(u == g.adj), "self-loop"
+    key = u.astype(np.int64) * g.n + g.adj
+    assert np.unique(key).shape[0] == g.m, "duplicate"
+    from oracle.exact import _components
+    nc, _ = _components(g.idx, g.adj)
+    assert nc == 1
+    assert g.degree().min() >= 1
+    for v in range(0, g.n, max(1, g.n // 50)):
+        row = g.adj[g.idx[v]:g.idx[v + 1]]
+        assert np.all(row[:-1] <= row[1:])
+
+
+def test_chung_lu_deterministic_simple_connected():
+    a = G.chung_lu(4096, 2.5, 22.0, seed=1)
+    b = G.chung_lu(4096, 2.5, 22.0, seed=1)
+    assert np.array_equal(a.idx, b.idx) and np.array_equal(a.adj, b.adj)
+    _simple_connected(a)
+    assert 85000 < a.m < 91000  # ~88k CSR entries
+    c = G.chung_lu(4096, 2.5, 22.0, seed=2)
+    assert not np.array_equal(a.adj[:1000], c.adj[:1000])
+
+
+def test_workload_cl4096_shape():
+    g = W.chung_lu_4096()
+    assert g.n == 4096 and 85000 < g.m < 91000 and g.degree().max() > 1000
+
+
+def test_grid_rewired_preserves_edge_count():
+    g = G.grid_rewired(40, 30, 0.05, seed=3)
+    E_ = 40 * 29 + 39 * 30
+    _simple_connected(g)
+    assert g.m >= 2 * E_ and g.m <= 2 * E_ + 2 * g.n  # repairs add at most one edge each
+    plain = G.grid_rewired(40, 30, 0.0, seed=3)
+    assert plain.m == 2 * E_ and plain.degree().max() == 4
+
+
+def test_write_txt_roundtrip(tmp_path):
+    g = G.chung_lu(500, 2.5, 9.0, seed=11)
+    p = tmp_path / "g.txt"
+    g.write_txt(p)
+    h = G.load_txt(p)
+    assert np.array_equal(g.idx, h.idx) and np.array_equal(g.adj, h.adj)
+
+
+def test_reverse_index_is_first_occurrence(tmp_path):
+    p = tmp_path / "g.txt"
+    p.write_text("0 1\n0 1\n1 2\n2 2\n")
+    g = G.load_txt(p)
+    rev = g.reverse_index()
+    u = np.repeat(np.arange(g.n), g.degree())
+    for pos in range(g.m):
+        v = g.adj[pos]
+        row = g.adj[g.idx[v]:g.idx[v + 1]]
+        assert rev[pos] == g.idx[v] + int(np.searchsorted(row, u[pos], side="left"))

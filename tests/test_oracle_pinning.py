@@ -1,0 +1,160 @@
+"""Pins the oracle against things other than itself: recorded known answers
+of the reference, published hash test vectors, closed forms and invariants.
+CPU only."""
+import json
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN
+from tgtp_amd import graph as G
+
+
+def _checksum(c):
+    c = c.astype(np.uint64)
+    return int(c.sum()), int((np.arange(1, c.shape[0] + 1, dtype=np.uint64) * c).sum())
+
+
+@pytest.mark.parametrize("case", json.loads((GOLDEN / "saba_kats.json").read_text())["cases"],
+                         ids=lambda c: f"{c['graph']}{c['n']}")
+def test_saba_known_answers(case):
+    """Recorded outputs of the reference's saba_randomwalk; both graphs have
+    power-of-two degrees, so every bouquet ends in the SplitMix32 fallback."""
+    g = G.cycle(case["n"]) if case["graph"] == "cycle" else G.complete(case["n"])
+    c = oracle.saba_walks(g.idx, g.adj, case["depth"], case["n_walks"], threads=1)
+    assert _checksum(c) == (case["visit_count"], case["checksum"])
+
+
+def test_fmix32_published_vectors():
+    # MurmurHash3_x86_32 of the empty key is fmix32(seed): SMHasher vectors
+    assert oracle.fmix32(0) == 0
+    assert oracle.fmix32(1) == 0x514E28B7
+    assert oracle.fmix32(0xFFFFFFFF) == 0x81F16F39
+
+
+def test_fmix32_inverse_with_published_constants():
+    """fmix32 is a bijection; undoing it with the published modular inverses
+    of its two multipliers pins both multipliers and all three shifts."""
+    inv1, inv2 = 0xA5CB9243, 0x7ED1B41D
+    assert (0x85EBCA6B * inv1) % 2**32 == 1
+    assert (0xC2B2AE35 * inv2) % 2**32 == 1
+
+    def unfmix(h):
+        h ^= h >> 16
+        h = (h * inv2) % 2**32
+        h ^= (h >> 13) ^ (h >> 26)
+        h = (h * inv1) % 2**32
+        h ^= h >> 16
+        return h
+
+    rng = np.random.default_rng(7)
+    for x in rng.integers(0, 2**32, size=2000, dtype=np.uint64).tolist():
+        assert unfmix(oracle.fmix32(x)) == x
+
+
+def test_visit_count_invariant_and_isolated_vertices():
+    # vertices 3 and 7 isolated: skipped as sources, never reached
+    g = G.Graph.from_edges(9, [(0, 1), (1, 2), (2, 0), (4, 5), (5, 6), (6, 8), (8, 4), (1, 4)])
+    deg = g.degree()
+    assert deg[3] == 0 and deg[7] == 0
+    for depth, w in [(5, 64), (15, 8), (33, 256)]:
+        c = oracle.saba_walks(g.idx, g.adj, depth, w)
+        assert int(c.sum()) == int((deg > 0).sum()) * w * depth
+        assert c[3] == 0 and c[7] == 0
+
+
+def test_star_closed_form():
+    """Leaves have degree 1, so every step from a leaf returns to the hub and
+    walks alternate hub/leaf whatever the hash picks."""
+    n, W = 17, 64
+    g = G.Graph.from_edges(n, [(0, i) for i in range(1, n)])
+    for L in (1, 4, 7):
+        c = oracle.saba_walks(g.idx, g.adj, L, W)
+        hub = W * (L // 2) + (n - 1) * W * ((L + 1) // 2)
+        assert int(c[0]) == hub
+        assert int(c.sum()) == n * W * L
+
+
+def test_thread_count_and_chunk_do_not_change_counters():
+    g = G.chung_lu(3000, 2.5, 12.0, seed=9)
+    a = oracle.saba_walks(g.idx, g.adj, 10, 128, threads=1)
+    b = oracle.saba_walks(g.idx, g.adj, 10, 128, threads=4, chunk=7)
+    assert np.array_equal(a, b)
+
+
+def test_source_ranges_add_up():
+    g = G.chung_lu(1500, 2.5, 10.0, seed=2)
+    full = oracle.saba_walks(g.idx, g.adj, 6, 64)
+    part = oracle.saba_walks(g.idx, g.adj, 6, 64, 0, 700).astype(np.uint64)
+    part += oracle.saba_walks(g.idx, g.adj, 6, 64, 700, 1500)
+    assert np.array_equal(full.astype(np.uint64), part)
+
+
+def test_walk2_hashed_phase_equals_two_one_way_walks():
+    """With all degrees odd no secret can reach zero, so the 2-way walk never
+    falls back and each walker follows the 1-way walk from its endpoint on
+    the same grid.  Integer-valued pvec keeps the float sum exact, so
+    x = sum_u pvec[u] * (count_s[u] - count_v[u])."""
+    g = G.petersen()  # 3-regular
+    pvec = np.arange(1, g.n + 1, dtype=np.float32)
+    for s, v, nr, L in [(0, 1, 16, 5), (0, 5, 64, 9), (3, 8, 8, 30)]:
+        cs = oracle.saba_walk_source(g.idx, g.adj, s, L, nr).astype(np.int64)
+        cv = oracle.saba_walk_source(g.idx, g.adj, v, L, nr).astype(np.int64)
+        x = oracle.walk2_saba(g.idx, g.adj, s, v, nr, L, pvec)
+        assert x == float((pvec.astype(np.int64) * (cs - cv)).sum())
+
+
+def test_walk2_identical_endpoints_cancel_before_fallback():
+    g = G.petersen()
+    pvec = np.linspace(0.1, 1.0, g.n, dtype=np.float32)
+    assert oracle.walk2_saba(g.idx, g.adj, 4, 4, 32, 12, pvec) == 0.0
+
+
+def test_pow_square_compiles_to_product():
+    """aesc.h:498 squares with std::pow(x, 2).  g++ folds that to x*x (the
+    correctly rounded square), so an IEEE build of the reference -- and this
+    oracle -- never calls glibc pow, which is NOT always correctly rounded
+    (e.g. pow(9413.715629082753, 2) != x*x).  The device computes x*x too.
+    This pins that no pow call survives in the oracle or the reference build."""
+    import shutil
+    import subprocess
+
+    x = 9413.715629082753
+    assert math.pow(x, 2) != x * x  # glibc pow differs here: the fold matters
+    objdump = shutil.which("objdump")
+    if objdump is None:
+        pytest.skip("objdump not available")
+    bins = [oracle.LIB_PATH] + ([oracle.REF_TGTP_MT] if oracle.REF_TGTP_MT.exists() else [])
+    for b in bins:
+        text = subprocess.run([objdump, "-d", str(b)], capture_output=True, text=True).stdout
+        assert "<pow@plt>" not in text and "<log@plt>" in text
+
+
+def test_exact_closed_forms():
+    from oracle.exact import exact_resistance
+
+    for n in (5, 8):
+        g = G.complete(n)
+        R, _ = exact_resistance(g.idx, g.adj)
+        assert np.allclose(R, 2.0 / n, rtol=0, atol=1e-12)
+    for n in (9, 15):
+        g = G.cycle(n)
+        R, _ = exact_resistance(g.idx, g.adj)
+        assert np.allclose(R, (n - 1) / n, rtol=0, atol=1e-12)
+    g = G.petersen()
+    R, _ = exact_resistance(g.idx, g.adj)
+    assert np.allclose(R, 0.6, rtol=0, atol=1e-12)
+
+
+def test_exact_foster_and_cg_agree():
+    from oracle.exact import exact_resistance, sampled_cg
+
+    g = G.grid_rewired(20, 20, 0.05, seed=4)
+    R, info = exact_resistance(g.idx, g.adj)
+    assert info["foster_error"] < 1e-8
+    assert abs(R.sum() / 2 - (g.n - 1)) < 1e-8
+    pos = np.arange(0, g.m, 97)
+    Rc = sampled_cg(g.idx, g.adj, pos)
+    assert np.allclose(Rc, R[pos], rtol=1e-9, atol=1e-11)

@@ -1,0 +1,14 @@
+"""tgtp_amd -- MI355X-native TGT+/SABA spanning-edge-centrality hot path.
+
+Layers:
+  include/tgtp_hip.h, include/tgtp_graph_io.h   C ABI (the drop-in boundary)
+  tgtp_amd/csrc/*.hip, *.cpp                     HIP kernels for gfx950 + host C++
+  tgtp_amd/libtgtp_hip.so                        built in-tree by tgtp_amd.build
+  tgtp_amd.runtime                               Context / DeviceGraph (ctypes)
+  tgtp_amd.aesc                                  the reference's AESC interface
+  tgtp_amd/bin/{tgtp,benchmark,rw_experiment_saba}  CLI drop-ins
+"""
+from ._lib import (MODE_FAITHFUL, MODE_FAST_BF16, MODE_FAST_FP32, TgtpError)  # noqa: F401
+from .graph import (Graph, chung_lu, complete, cycle, grid_rewired, load_eigens_txt,  # noqa: F401
+                    load_txt, petersen)
+from .runtime import Context, DeviceGraph  # noqa: F401

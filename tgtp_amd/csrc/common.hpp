@@ -1,0 +1,102 @@
+// common.hpp -- shared pieces of the HIP library: the context, device CSR,
+// status plumbing and the integer mixers used by every walk kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "tgtp_hip.h"
+
+// ---- integer mixers --------------------------------------------------------
+// MurmurHash3 fmix32 (aesc.h:31-41).  Applied to the current vertex id; the
+// result is xored into the walk's grid secret to pick the next neighbour.
+__host__ __device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+
+// SplitMix32 16/15/15 (aesc.h:23-30): the fallback generator, one state per
+// walk, seeded with the walk's initial grid secret.
+__host__ __device__ __forceinline__ uint32_t splitmix32(uint32_t& state) {
+  state += 0x9e3779b9u;
+  uint32_t z = state ^ (state >> 16);
+  z *= 0x21f0aaadu;
+  z ^= z >> 15;
+  z *= 0x735a2d97u;
+  return z ^ (z >> 15);
+}
+
+// ---- device-resident CSR ----------------------------------------------------
+// rowd[v] = {idx[v], deg(v)}: one aligned 8-byte load per hop instead of two
+// dependent 4-byte loads of idx[v] and idx[v+1].
+struct tgtp_dgraph {
+  int device = -1;
+  uint64_t n = 0, m = 0;
+  uint32_t* idx = nullptr;  // [n+1]
+  uint32_t* adj = nullptr;  // [m]
+  uint2* rowd = nullptr;    // [n]
+  uint32_t* rev = nullptr;  // [m], built on first tgtp use
+  float* degf = nullptr;    // [n], float(deg)
+  // cached eigenpairs
+  const void* eig_key = nullptr;
+  int32_t eig_omega = 0;
+  double* lam = nullptr;    // [omega]
+  float* X = nullptr;       // [n*omega]
+};
+
+// ---- context -----------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct tgtp_ctx {
+  int device = 0;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8] = {};
+  std::string err;
+  DevBuf bufs[16];
+};
+
+// Scratch slots in tgtp_ctx::bufs.
+enum BufSlot {
+  kBufCounters = 0,
+  kBufSums = 1,
+  kBufTaus = 2,
+  kBufPreds = 3,
+  kBufPvec = 4,
+  kBufQueue = 5,
+  kBufSrcInfo = 6,
+  kBufEdgeInfo = 7,
+  kBufStage = 8,
+  kBufMisc = 9,
+  kBufFlags = 10,
+};
+
+// Returns a device buffer of at least `bytes` in slot `slot` (grow-only).
+void* ctx_buffer(tgtp_ctx* ctx, int slot, size_t bytes);
+
+int set_error(tgtp_ctx* ctx, int code, const std::string& msg);
+
+#define TGTP_HIP_CHECK(ctx, expr)                                                   \
+  do {                                                                              \
+    hipError_t _e = (expr);                                                         \
+    if (_e != hipSuccess)                                                           \
+      return set_error((ctx), TGTP_E_HIP,                                           \
+                       std::string(#expr) + ": " + hipGetErrorString(_e));          \
+  } while (0)
+
+// ---- kernel launchers (defined in the .hip files) ----------------------------
+int launch_saba_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,
+                      uint32_t src_end, uint32_t depth, uint32_t n_walks,
+                      uint32_t* d_counters, unsigned long long* d_sums,
+                      float* walk_ms);

@@ -1,0 +1,124 @@
+"""Context / device-graph wrappers over the C ABI (include/tgtp_hip.h)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .graph import Graph
+
+
+def _check(ctx_ptr, rc: int, what: str) -> None:
+    if rc != L.TGTP_OK:
+        lib = L.lib()
+        msg = lib.tgtp_ctx_last_error(ctx_ptr).decode() if ctx_ptr else ""
+        raise L.TgtpError(rc, f"{what}: {lib.tgtp_strerror(rc).decode()}: {msg}")
+
+
+class DeviceGraph:
+    """A CSR resident in HBM (tgtp_graph_upload)."""
+
+    def __init__(self, ctx: "Context", g: Graph):
+        self.ctx = ctx
+        self.n, self.m = g.n, g.m
+        self.host = g
+        h = C.c_void_p()
+        gs = g.struct()
+        _check(ctx.ptr, L.lib().tgtp_graph_upload(ctx.ptr, C.byref(gs), C.byref(h)),
+               "tgtp_graph_upload")
+        self.ptr = h
+
+    def free(self) -> None:
+        if self.ptr:
+            L.lib().tgtp_graph_free(self.ctx.ptr, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One HIP stream + scratch on one device (tgtp_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        rc = L.lib().tgtp_ctx_create(device, C.byref(h))
+        if rc != L.TGTP_OK:
+            raise L.TgtpError(rc, f"tgtp_ctx_create({device}): "
+                                  f"{L.lib().tgtp_strerror(rc).decode()}")
+        self.ptr = h
+        self.device = device
+
+    def close(self) -> None:
+        if self.ptr:
+            L.lib().tgtp_ctx_destroy(self.ptr)
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, g: Graph) -> DeviceGraph:
+        return DeviceGraph(self, g)
+
+    def saba_walks(self, dg: DeviceGraph, depth: int, n_walks: int, src_begin: int = 0,
+                   src_end: int | None = None, want_counters: bool = True):
+        """-> (counters uint32[n] or None, visit_count, checksum, stats dict)."""
+        src_end = dg.n if src_end is None else src_end
+        cnt = np.zeros(dg.n, dtype=np.uint32) if want_counters else None
+        vc, cs = C.c_uint64(), C.c_uint64()
+        st = L.StatsStruct()
+        rc = L.lib().tgtp_saba_walks(self.ptr, dg.ptr, src_begin, src_end, depth, n_walks,
+                                     L.ptr(cnt, C.c_uint32) if cnt is not None else None,
+                                     C.byref(vc), C.byref(cs), C.byref(st))
+        _check(self.ptr, rc, "tgtp_saba_walks")
+        return cnt, int(vc.value), int(cs.value), st.as_dict()
+
+    @staticmethod
+    def _eigens(lam: np.ndarray, X: np.ndarray):
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        return L.EigensStruct(lam.shape[0], L.ptr(lam, C.c_double), L.ptr(X, C.c_float)), (lam, X)
+
+    def taus(self, g: Graph, lam, X, eps_half: float, mode: int = L.MODE_FAITHFUL):
+        es, keep = self._eigens(lam, X)
+        out = np.zeros(g.m, dtype=np.int32)
+        st = L.StatsStruct()
+        gs = g.struct()
+        rc = L.lib().tgtp_taus(self.ptr, C.byref(gs), C.byref(es), float(np.float32(eps_half)),
+                               mode, L.ptr(out, C.c_int32), C.byref(st))
+        _check(self.ptr, rc, "tgtp_taus")
+        del keep
+        return out, st.as_dict()
+
+    def score(self, g, lam, X, epsilon: float, gamma: int = 10, delta: float | None = None,
+              mode: int = L.MODE_FAITHFUL):
+        """tgtp() scores float32[m] (CSR order) and stats.  `g` is a Graph
+        (copied to HBM per call) or a DeviceGraph."""
+        es, keep = self._eigens(lam, X)
+        n = g.n
+        ps = L.ParamsStruct(epsilon, (1.0 / n) if delta is None else delta, gamma, mode)
+        out = np.zeros(g.m, dtype=np.float32)
+        st = L.StatsStruct()
+        if isinstance(g, DeviceGraph):
+            rc = L.lib().tgtp_score_dev(self.ptr, g.ptr, C.byref(es), C.byref(ps),
+                                        L.ptr(out, C.c_float), C.byref(st))
+        else:
+            gs = g.struct()
+            rc = L.lib().tgtp_score(self.ptr, C.byref(gs), C.byref(es), C.byref(ps),
+                                    L.ptr(out, C.c_float), C.byref(st))
+        _check(self.ptr, rc, "tgtp_score")
+        del keep
+        return out, st.as_dict()
