@@ -137,6 +137,20 @@ int tgtp_score(tgtp_ctx* ctx, const tgtp_graph* g, const tgtp_eigens* e,
 int tgtp_score_dev(tgtp_ctx* ctx, const tgtp_dgraph* dg, const tgtp_eigens* e,
                    const tgtp_params* p, float* scores_out, tgtp_stats* stats);
 
+/* Diagnostic form of tgtp_score_dev that also returns the per-stage values
+ * of the pipeline (each output pointer may be NULL): taus [m] (calculate_taus,
+ * aesc.h:164-216), nr [m] walk count per position (0 where tau <= ell,
+ * aesc.h:519-521), walk_sum [m] raw 2-way walk sum (aesc.h:523),
+ * preds_row [m] row predictions before symmetrization (aesc.h:532), and per
+ * source ell / edge_max / global max / global min / final frontier size
+ * [n] (aesc.h:460-506).  No counterpart in the reference: it exposes the
+ * intermediate values of AESC::tgtp for stage-by-stage parity checks. */
+int tgtp_score_trace(tgtp_ctx* ctx, const tgtp_dgraph* dg, const tgtp_eigens* e,
+                     const tgtp_params* p, float* scores_out, tgtp_stats* stats,
+                     int32_t* taus, uint64_t* nr, float* walk_sum, float* preds_row,
+                     int32_t* ell, float* edge_max, float* gmax, float* gmin,
+                     uint32_t* q_size);
+
 #ifdef __cplusplus
 }
 #endif

@@ -296,7 +296,7 @@ extern "C" int orc_taus(uint64_t n, uint64_t m, const uint32_t* idx,
                         const uint32_t* adj, int omega, const double* lam,
                         const float* X, float eps_half, int32_t* taus,
                         int threads) {
-  if (omega < 3) return -1;
+  if (omega < 2) return -1;
   const Csr g{n, m, idx, adj};
   all_taus(g, X, omega, lam, eps_half, taus, threads);
   return 0;
@@ -307,8 +307,8 @@ extern "C" int orc_tgtp(uint64_t n, uint64_t m, const uint32_t* idx,
                         const float* X, double epsilon, double delta, int gamma,
                         int walk_mode, int threads, float* scores,
                         orc_tgtp_trace* trace) {
-  if (gamma < 1 || omega < 3) {
-    g_err = "require gamma >= 1 and omega >= 3";
+  if (gamma < 1 || omega < 2) {
+    g_err = "require gamma >= 1 and omega >= 2";
     return -1;
   }
   const Csr g{n, m, idx, adj};

@@ -1,0 +1,111 @@
+"""Command-line drop-ins: argument validation and error messages follow the
+reference CLIs (main.cpp:9-32, benchmark.cpp:10-28,
+rw_experiment_saba.cpp:150-165); GPU-marked tests check their outputs
+against the oracle."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from tgtp_amd import eigenprep as E
+from tgtp_amd import graph as G
+
+BIN = Path(__file__).resolve().parents[1] / "tgtp_amd" / "bin"
+
+
+def run(args, cwd=None):
+    return subprocess.run([str(a) for a in args], capture_output=True, text=True, cwd=cwd)
+
+
+@pytest.mark.parametrize("argv,msg", [
+    (["d", "g", "0", "128", "10", "o"], "require 0 < epsilon < 1"),
+    (["d", "g", "1.5", "128", "10", "o"], "require 0 < epsilon < 1"),
+    (["d", "g", "0.05", "1", "10", "o"], "integer omega >= 2"),
+    (["d", "g", "0.05", "12.5", "10", "o"], "integer omega"),
+    (["d", "g", "0.05", "128", "0", "o"], "gamma >= 1"),
+    (["d", "g", "0.05x", "128", "10", "o"], "invalid numeric argument"),
+])
+def test_tgtp_cli_validation(argv, msg):
+    r = run([BIN / "tgtp", *argv])
+    assert r.returncode == 1 and msg in r.stderr and r.stderr.startswith("tgtp: ")
+
+
+def test_tgtp_cli_usage_and_missing_graph(tmp_path):
+    r = run([BIN / "tgtp"])
+    assert r.returncode == 1 and "Usage" in r.stderr
+    r = run([BIN / "tgtp", tmp_path, "nope", "0.05", "128", "10", tmp_path / "o.bin"])
+    assert r.returncode == 1 and "cannot open graph" in r.stderr
+
+
+def test_benchmark_cli_validation(tmp_path):
+    assert "Usage" in run([BIN / "benchmark"]).stderr
+    for args, msg in [(["g", "0.05,"], "empty CSV"), (["g", "0.05,,0.01"], "empty CSV"),
+                      (["g", "2"], "epsilon must be between"), (["g", "0.05", "0"], "positive integer"),
+                      (["g", "0.05", "1", "3", str(tmp_path), "128", "10", "-1"], "invalid warmups")]:
+        r = run([BIN / "benchmark", *args])
+        assert r.returncode == 1 and msg in r.stderr, (args, r.stderr)
+
+
+def test_rw_cli_validation():
+    r = run([BIN / "rw_experiment_saba", "d", "g", "5", "12"])
+    assert r.returncode == 1 and "multiple of 8" in r.stderr
+    assert "Usage" in run([BIN / "rw_experiment_saba"]).stderr
+
+
+def _dataset(root: Path, name: str, g, omega: int):
+    d = root / name
+    d.mkdir(parents=True)
+    g.write_txt(d / "graph.txt")
+    lam, X = E.eigens_f32(*E.compute_eigens(g, omega))
+    E.write_eigens_txt(d / f"sorted_eigens_{omega}.txt", lam, X)
+    return lam, X
+
+
+@pytest.mark.gpu
+def test_tgtp_cli_scores_bin_matches_oracle(tmp_path):
+    g = G.chung_lu(800, 2.5, 10.0, seed=12)
+    lam, X = _dataset(tmp_path, "cl800", g, 32)
+    out = tmp_path / "scores.bin"
+    r = run([BIN / "tgtp", tmp_path, "cl800", "0.05", "32", "10", out])
+    assert r.returncode == 0, r.stderr
+    s = np.fromfile(out, dtype=np.float32)
+    assert s.shape == (g.m,)
+    ref = oracle.tgtp(g.idx, g.adj, lam, X, 0.05, 10)
+    assert np.array_equal(s.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_benchmark_cli_rows(tmp_path):
+    g = G.chung_lu(600, 2.5, 9.0, seed=13)
+    _dataset(tmp_path, "cl600", g, 16)
+    r = run([BIN / "benchmark", "cl600", "0.05,0.1", "1", "2", tmp_path, "16", "10", "1"])
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    assert lines[0] == "sampler\tdataset\tepsilon\tthreads\trepeat\tseconds\tchecksum"
+    rows = [ln.split("\t") for ln in lines[1:]]
+    assert len(rows) == 4 and all(len(x) == 7 and x[0] == "seed32-grid-multiply-high" for x in rows)
+    # deterministic: both repeats of one epsilon print the same checksum
+    assert rows[0][6] == rows[1][6] and rows[2][6] == rows[3][6]
+
+
+@pytest.mark.gpu
+def test_rw_cli_tsv_row(tmp_path):
+    g = G.chung_lu(1000, 2.5, 12.0, seed=14)
+    d = tmp_path / "data" / "cl1000"
+    d.mkdir(parents=True)
+    g.write_txt(d / "graph.txt")
+    r = run([BIN / "rw_experiment_saba", tmp_path / "data", "cl1000", "5", "64", "4"], cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    cols = r.stdout.rstrip("\n").split("\t")
+    assert len(cols) == 14 and cols[0] == "saba" and cols[1] == "cl1000"
+    assert cols[2:7] == ["5", "64", "4", str(g.n), str(g.m)]
+    cnt = oracle.saba_walks(g.idx, g.adj, 5, 64).astype(np.uint64)
+    vc = float(cnt.sum())
+    cs = float((np.arange(1, g.n + 1, dtype=np.uint64) * cnt).sum())
+    # default ostream formatting, 6 significant digits (rw_experiment_saba.cpp:241-244)
+    assert cols[12] == f"{vc:g}" and cols[13] == f"{cs:g}"
+    res = (tmp_path / "results" / "rw" / "saba_cl1000_d5_w64.txt").read_text().splitlines()
+    kv = dict(line.split(" ", 1) for line in res)
+    assert int(kv["visit_count_exact"]) == int(cnt.sum())

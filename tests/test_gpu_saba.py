@@ -64,8 +64,10 @@ def test_bouquets_spanning_sources_in_a_wave(gpu_ctx, W_):
 
 
 def test_isolated_vertices_and_partial_ranges(gpu_ctx):
-    g = G.Graph.from_edges(40, [(i, (i * 7 + 3) % 40) for i in range(40) if i % 5 != 0] + [(1, 2)])
-    assert (g.degree() == 0).any()
+    # vertices 0, 20 and 41 have no edges
+    edges = [(i, (i * 7 + 3) % 40) for i in range(1, 40) if i % 5 != 0 and (i * 7 + 3) % 40 not in (0, 20)]
+    g = G.Graph.from_edges(42, [e for e in edges if 20 not in e] + [(1, 2)])
+    assert g.degree(0) == 0 and g.degree(20) == 0 and g.degree(41) == 0
     _check(gpu_ctx, g, 12, 64, 3, 37, label="isolated")
 
 

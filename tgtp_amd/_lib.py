@@ -65,7 +65,7 @@ class StatsStruct(C.Structure):
 EXPORTED = [
     "tgtp_ctx_create", "tgtp_ctx_destroy", "tgtp_ctx_last_error", "tgtp_strerror",
     "tgtp_graph_upload", "tgtp_graph_free", "tgtp_saba_walks", "tgtp_taus",
-    "tgtp_score", "tgtp_score_dev",
+    "tgtp_score", "tgtp_score_dev", "tgtp_score_trace",
     "tgtp_load_txt", "tgtp_csr_free", "tgtp_load_eigens_txt", "tgtp_check_aesc_graph",
     "tgtp_gen_chung_lu", "tgtp_gen_grid_rewired", "tgtp_write_graph_txt",
     "tgtp_csr_reverse_index",
@@ -101,6 +101,10 @@ def lib() -> C.CDLL:
                              P(C.c_float), P(StatsStruct)]
     L.tgtp_score_dev.argtypes = [vp, vp, P(EigensStruct), P(ParamsStruct),
                                  P(C.c_float), P(StatsStruct)]
+    L.tgtp_score_trace.argtypes = [vp, vp, P(EigensStruct), P(ParamsStruct), P(C.c_float),
+                                   P(StatsStruct), P(C.c_int32), P(C.c_uint64), P(C.c_float),
+                                   P(C.c_float), P(C.c_int32), P(C.c_float), P(C.c_float),
+                                   P(C.c_float), P(C.c_uint32)]
     L.tgtp_load_txt.argtypes = [C.c_char_p, P(CsrStruct), C.c_char_p, C.c_size_t]
     L.tgtp_csr_free.argtypes = [P(CsrStruct)]
     L.tgtp_csr_free.restype = None

@@ -165,6 +165,7 @@ extern "C" void tgtp_graph_free(tgtp_ctx* ctx, tgtp_dgraph* dg) {
   if (!dg) return;
   if (ctx) (void)hipStreamSynchronize(ctx->stream);
   (void)hipSetDevice(dg->device);
+  if (dg->prep && dg->prep_free) dg->prep_free(dg->prep);
   void* ptrs[] = {dg->idx, dg->adj, dg->rowd, dg->rev, dg->degf, dg->lam, dg->X};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

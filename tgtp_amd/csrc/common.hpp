@@ -45,6 +45,9 @@ struct tgtp_dgraph {
   uint2* rowd = nullptr;    // [n]
   uint32_t* rev = nullptr;  // [m], built on first tgtp use
   float* degf = nullptr;    // [n], float(deg)
+  // tgtp host-side state (host CSR copy, symmetrization map), built lazily
+  void* prep = nullptr;
+  void (*prep_free)(void*) = nullptr;
   // cached eigenpairs
   const void* eig_key = nullptr;
   int32_t eig_omega = 0;
@@ -80,6 +83,7 @@ enum BufSlot {
   kBufStage = 8,
   kBufMisc = 9,
   kBufFlags = 10,
+  kBufTrace = 11,
 };
 
 // Returns a device buffer of at least `bytes` in slot `slot` (grow-only).

@@ -122,3 +122,28 @@ class Context:
         _check(self.ptr, rc, "tgtp_score")
         del keep
         return out, st.as_dict()
+
+    def score_trace(self, dg: DeviceGraph, lam, X, epsilon: float, gamma: int = 10,
+                    delta: float | None = None, mode: int = L.MODE_FAITHFUL):
+        """tgtp_score_trace: scores plus the per-stage arrays (same keys as
+        the oracle's trace)."""
+        es, keep = self._eigens(lam, X)
+        n, m = dg.n, dg.m
+        ps = L.ParamsStruct(epsilon, (1.0 / n) if delta is None else delta, gamma, mode)
+        out = np.zeros(m, dtype=np.float32)
+        tr = dict(taus=np.zeros(m, np.int32), nr=np.zeros(m, np.uint64),
+                  walk_sum=np.zeros(m, np.float32), preds_row=np.zeros(m, np.float32),
+                  ell=np.zeros(n, np.int32), edge_max=np.zeros(n, np.float32),
+                  gmax=np.zeros(n, np.float32), gmin=np.zeros(n, np.float32),
+                  q_size=np.zeros(n, np.uint32))
+        ct = {np.int32: C.c_int32, np.uint64: C.c_uint64, np.float32: C.c_float,
+              np.uint32: C.c_uint32}
+        args = [L.ptr(tr[k], ct[tr[k].dtype.type]) for k in
+                ("taus", "nr", "walk_sum", "preds_row", "ell", "edge_max", "gmax", "gmin",
+                 "q_size")]
+        st = L.StatsStruct()
+        rc = L.lib().tgtp_score_trace(self.ptr, dg.ptr, C.byref(es), C.byref(ps),
+                                      L.ptr(out, C.c_float), C.byref(st), *args)
+        _check(self.ptr, rc, "tgtp_score_trace")
+        del keep
+        return out, tr, st.as_dict()
