@@ -96,6 +96,60 @@ def bench_saba(args):
     return value, ms_per_step, roof, cfg, g
 
 
+def bench_tgtp(args):
+    """tgtp() on the n=4096 Chung-Lu graph (omega=128, gamma=10): edges/s =
+    CSR entries / seconds of one call with graph and eigenpairs resident in
+    HBM (benchmark.cpp:40-42 times tgtp() only).  Returns a dict of legs."""
+    import numpy as np
+
+    import oracle
+    from tgtp_amd import MODE_FAITHFUL, Context, eigenprep as E, workloads as WL
+
+    g = WL.chung_lu_4096()
+    t0 = time.time()
+    lam, X = E.eigens_f32(*E.compute_eigens(g, 128))
+    log(f"[bench] tgtp graph n={g.n} m={g.m}, eigenpairs in {time.time() - t0:.1f}s")
+    out = {"graph": WL.describe(WL.CL4096), "n": g.n, "m": g.m, "omega": 128, "gamma": 10}
+    threads = host_threads()
+    with Context(0) as ctx:
+        dg = ctx.upload(g)
+        for eps in (0.01, 0.05):
+            leg = {}
+            for mode, name in ((MODE_FAITHFUL, "faithful"),):
+                s, st = ctx.score(dg, lam, X, eps, 10, mode=mode)  # warmup + eigen upload
+                times, kms = [], []
+                for _ in range(max(3, args.steps)):
+                    t1 = time.perf_counter()
+                    s, st = ctx.score(dg, lam, X, eps, 10, mode=mode)
+                    times.append(time.perf_counter() - t1)
+                    kms.append(st["kernel_ms"])
+                best = min(times)
+                ref = oracle.tgtp(g.idx, g.adj, lam, X, eps, 10, threads=threads)
+                leg[name] = {"edges_per_s": g.m / best, "seconds": best,
+                             "kernel_ms": min(kms), "taus_ms": st["taus_ms"],
+                             "diffuse_ms": st["diffuse_ms"], "host_fixups": st["host_fixups"],
+                             "bitwise_equal_cpu": bool(np.array_equal(s.view(np.uint32),
+                                                                      ref.view(np.uint32)))}
+            # PCIe-inclusive: host arrays in, scores out (tgtp_score)
+            t1 = time.perf_counter()
+            ctx.score(g, lam, X, eps, 10)
+            leg["faithful"]["e2e_with_h2d_edges_per_s"] = g.m / (time.perf_counter() - t1)
+            ctimes = []
+            for _ in range(3):
+                t1 = time.perf_counter()
+                oracle.tgtp(g.idx, g.adj, lam, X, eps, 10, threads=threads)
+                ctimes.append(time.perf_counter() - t1)
+            leg["cpu_baseline"] = {"edges_per_s": g.m / min(ctimes), "seconds": min(ctimes),
+                                   "cores": threads, "kind": "port",
+                                   "sample": "full run, min of 3 (oracle C++ OpenMP, "
+                                             "schedule(dynamic) over sources)"}
+            out[f"eps_{eps}"] = leg
+            log(f"[bench] tgtp eps={eps}: faithful {leg['faithful']['edges_per_s']:.3e} edges/s "
+                f"vs cpu {leg['cpu_baseline']['edges_per_s']:.3e}")
+        dg.free()
+    return out
+
+
 def cpu_saba_baseline(g, depth, n_walks, sample_sources: int):
     """The oracle's OpenMP walk (schedule(dynamic) over sources, per-thread
     counters) on evenly spread source slices of the same workload."""
@@ -129,6 +183,7 @@ def main() -> int:
     ap.add_argument("--walks", type=int, default=16384)
     ap.add_argument("--cpu-sample-sources", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tgtp", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
@@ -146,6 +201,9 @@ def main() -> int:
         line["cpu_baseline"] = cpu_saba_baseline(g, args.depth, args.walks,
                                                  args.cpu_sample_sources)
         line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+    del g
+    if not args.no_tgtp:
+        line["tgtp"] = bench_tgtp(args)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -87,6 +87,15 @@ def test_rewired_grid(gpu_ctx):
     _check(gpu_ctx, G.grid_rewired(64, 64, 0.05, seed=1), 10, 256, label="grid64")
 
 
+@pytest.mark.parametrize("packed,hot", [("0", "32768"), ("1", "0"), ("0", "100"), ("1", "4096")])
+def test_layouts_and_lds_staging_agree(gpu_ctx, monkeypatch, packed, hot):
+    """The packed {id|start|degree} walk layout and the plain CSR give the
+    same counters, for any LDS-staged vertex range."""
+    monkeypatch.setenv("TGTP_WALK_PACKED", packed)
+    monkeypatch.setenv("TGTP_WALK_HOT", hot)
+    _check(gpu_ctx, W.chung_lu_4096(), 9, 512, 5, 4000, label=f"packed={packed} hot={hot}")
+
+
 def test_deterministic_repeat(gpu_ctx):
     g = W.chung_lu_4096()
     dg = gpu_ctx.upload(g)

@@ -139,6 +139,8 @@ extern "C" int tgtp_graph_upload(tgtp_ctx* ctx, const tgtp_graph* g, tgtp_dgraph
   dg->device = ctx->device;
   dg->n = g->n;
   dg->m = g->m;
+  for (uint64_t v = 0; v < g->n; ++v)
+    dg->max_deg = std::max<uint64_t>(dg->max_deg, g->idx[v + 1] - g->idx[v]);
   auto fail = [&](const char* what) {
     tgtp_graph_free(ctx, dg);
     return set_error(ctx, TGTP_E_NOMEM, std::string("device allocation: ") + what);
@@ -166,7 +168,7 @@ extern "C" void tgtp_graph_free(tgtp_ctx* ctx, tgtp_dgraph* dg) {
   if (ctx) (void)hipStreamSynchronize(ctx->stream);
   (void)hipSetDevice(dg->device);
   if (dg->prep && dg->prep_free) dg->prep_free(dg->prep);
-  void* ptrs[] = {dg->idx, dg->adj, dg->rowd, dg->rev, dg->degf, dg->lam, dg->X};
+  void* ptrs[] = {dg->idx, dg->adj, dg->rowd, dg->rev, dg->degf, dg->lam, dg->X, dg->walk_adj};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete dg;

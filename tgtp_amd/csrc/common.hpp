@@ -45,6 +45,11 @@ struct tgtp_dgraph {
   uint2* rowd = nullptr;    // [n]
   uint32_t* rev = nullptr;  // [m], built on first tgtp use
   float* degf = nullptr;    // [n], float(deg)
+  uint64_t max_deg = 0;
+  // packed walk layout {id | row start | degree} per CSR entry, built by the
+  // first walk call when the three fields fit in 64 bits
+  uint64_t* walk_adj = nullptr;  // [m]
+  uint32_t id_bits = 0, start_bits = 0;
   // tgtp host-side state (host CSR copy, symmetrization map), built lazily
   void* prep = nullptr;
   void (*prep_free)(void*) = nullptr;
