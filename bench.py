@@ -103,19 +103,25 @@ def bench_tgtp(args):
     import numpy as np
 
     import oracle
-    from tgtp_amd import MODE_FAITHFUL, Context, eigenprep as E, workloads as WL
+    from oracle.exact import exact_resistance
+    from tgtp_amd import (MODE_FAITHFUL, MODE_FAST_BF16, MODE_FAST_FP32, Context,
+                          eigenprep as E, workloads as WL)
 
     g = WL.chung_lu_4096()
     t0 = time.time()
     lam, X = E.eigens_f32(*E.compute_eigens(g, 128))
-    log(f"[bench] tgtp graph n={g.n} m={g.m}, eigenpairs in {time.time() - t0:.1f}s")
+    R, _ = exact_resistance(g.idx, g.adj)
+    log(f"[bench] tgtp graph n={g.n} m={g.m}, eigenpairs + exact oracle in {time.time() - t0:.1f}s")
     out = {"graph": WL.describe(WL.CL4096), "n": g.n, "m": g.m, "omega": 128, "gamma": 10}
     threads = host_threads()
+    modes = ((MODE_FAITHFUL, "faithful"), (MODE_FAST_FP32, "fast_fp32"),
+             (MODE_FAST_BF16, "fast_bf16"))
     with Context(0) as ctx:
         dg = ctx.upload(g)
         for eps in (0.01, 0.05):
             leg = {}
-            for mode, name in ((MODE_FAITHFUL, "faithful"),):
+            ref = oracle.tgtp(g.idx, g.adj, lam, X, eps, 10, threads=threads)
+            for mode, name in modes:
                 s, st = ctx.score(dg, lam, X, eps, 10, mode=mode)  # warmup + eigen upload
                 times, kms = [], []
                 for _ in range(max(3, args.steps)):
@@ -124,12 +130,17 @@ def bench_tgtp(args):
                     times.append(time.perf_counter() - t1)
                     kms.append(st["kernel_ms"])
                 best = min(times)
-                ref = oracle.tgtp(g.idx, g.adj, lam, X, eps, 10, threads=threads)
+                err = np.abs(s.astype(np.float64) - R)
                 leg[name] = {"edges_per_s": g.m / best, "seconds": best,
                              "kernel_ms": min(kms), "taus_ms": st["taus_ms"],
                              "diffuse_ms": st["diffuse_ms"], "host_fixups": st["host_fixups"],
                              "bitwise_equal_cpu": bool(np.array_equal(s.view(np.uint32),
-                                                                      ref.view(np.uint32)))}
+                                                                      ref.view(np.uint32))),
+                             "vs_exact_fp64": {"max_abs": float(err.max()),
+                                               "mean_abs": float(err.mean()),
+                                               "max_rel": float((err / R).max()),
+                                               "mean_rel": float((err / R).mean()),
+                                               "within_eps": bool(err.max() <= eps)}}
             # PCIe-inclusive: host arrays in, scores out (tgtp_score)
             t1 = time.perf_counter()
             ctx.score(g, lam, X, eps, 10)
@@ -144,8 +155,9 @@ def bench_tgtp(args):
                                    "sample": "full run, min of 3 (oracle C++ OpenMP, "
                                              "schedule(dynamic) over sources)"}
             out[f"eps_{eps}"] = leg
-            log(f"[bench] tgtp eps={eps}: faithful {leg['faithful']['edges_per_s']:.3e} edges/s "
-                f"vs cpu {leg['cpu_baseline']['edges_per_s']:.3e}")
+            log(f"[bench] tgtp eps={eps}: " + ", ".join(
+                f"{nm} {leg[nm]['edges_per_s']:.3e} edges/s (max err {leg[nm]['vs_exact_fp64']['max_abs']:.4f})"
+                for _, nm in modes) + f"; cpu {leg['cpu_baseline']['edges_per_s']:.3e}")
         dg.free()
     return out
 

@@ -1,0 +1,121 @@
+"""FAST_FP32 / FAST_BF16 tgtp on the GPU, through the C ABI.
+
+The fast modes change the arithmetic (synchronous diffusion, MFMA tau
+projection, FP64 row sums), so they are checked against the FP64 exact
+resistances (the accuracy oracle) and against the faithful path, not
+bitwise.  Tolerances are written in each test.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle.exact import exact_resistance
+from tgtp_amd import MODE_FAITHFUL, MODE_FAST_BF16, MODE_FAST_FP32
+from tgtp_amd import eigenprep as E
+from tgtp_amd import graph as G
+from tgtp_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+MODES = [(MODE_FAST_FP32, "fp32"), (MODE_FAST_BF16, "bf16")]
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def cl4096():
+    g = W.chung_lu_4096()
+    lam, X = E.eigens_f32(*E.compute_eigens(g, 128))
+    R, _ = exact_resistance(g.idx, g.adj)
+    return g, lam, X, R
+
+
+@pytest.mark.parametrize("eps", [0.05, 0.01])
+@pytest.mark.parametrize("mode,name", MODES)
+def test_fast_within_eps_of_exact(gpu_ctx, cl4096, eps, mode, name):
+    """|score - R_exact| <= eps on every CSR entry (the accuracy bound of
+    paper_accuracy.py:69-73), deterministic across calls."""
+    g, lam, X, R = cl4096
+    _say(f"fast {name} cl4096 eps={eps}")
+    dg = gpu_ctx.upload(g)
+    s1, st = gpu_ctx.score(dg, lam, X, eps, 10, mode=mode)
+    s2, _ = gpu_ctx.score(dg, lam, X, eps, 10, mode=mode)
+    f, _ = gpu_ctx.score(dg, lam, X, eps, 10, mode=MODE_FAITHFUL)
+    dg.free()
+    assert np.array_equal(s1.view(np.uint32), s2.view(np.uint32)), "not deterministic"
+    err = np.abs(s1.astype(np.float64) - R)
+    ferr = np.abs(f.astype(np.float64) - R)
+    rel = err / R
+    _say(f"  {name}: max abs {err.max():.5f} mean {err.mean():.6f} max rel {rel.max():.4f} "
+         f"mean rel {rel.mean():.5f} | faithful max abs {ferr.max():.5f} | kernel "
+         f"{st['kernel_ms']:.2f} ms (taus {st['taus_ms']:.2f})")
+    assert err.max() <= eps
+    # no worse than the faithful path's own error by more than eps/2
+    assert err.mean() <= ferr.mean() + eps / 2
+
+
+@pytest.mark.parametrize("mode,name", MODES)
+def test_mfma_taus_match_faithful(gpu_ctx, cl4096, mode, name):
+    """The projection changes only rounding: taus equal the faithful ones
+    except where the bound sits on an integer boundary (<= 1% FP32, <= 5%
+    BF16 of CSR positions)."""
+    g, lam, X, _ = cl4096
+    for eps in (0.05, 0.01):
+        tf, _ = gpu_ctx.taus(g, lam, X, np.float32(eps / 2), MODE_FAITHFUL)
+        tq, _ = gpu_ctx.taus(g, lam, X, np.float32(eps / 2), mode)
+        diff = np.mean(tf != tq)
+        _say(f"  taus {name} eps={eps}: {diff * 100:.3f}% differ, max |d| {np.abs(tf - tq).max()}")
+        assert diff <= (0.01 if mode == MODE_FAST_FP32 else 0.05)
+        assert np.abs(tf - tq).max() <= 2
+
+
+def test_fast_symmetric_output_and_trace(gpu_ctx):
+    g = G.chung_lu(1500, 2.5, 12.0, seed=31)
+    lam, X = E.eigens_f32(*E.compute_eigens(g, 32))
+    dg = gpu_ctx.upload(g)
+    s, tr, st = gpu_ctx.score_trace(dg, lam, X, 0.05, 10, mode=MODE_FAST_FP32)
+    dg.free()
+    rev = g.reverse_index()
+    assert np.array_equal(s, s[rev])
+    assert tr["ell"].min() >= 1 and np.all(tr["q_size"] >= 1)
+
+
+def test_fast_state_in_hbm(gpu_ctx):
+    """n beyond the LDS-resident limit: per-source state in HBM.  This sparse
+    graph has bridge edges (R = 1) that the tau truncation itself misses --
+    the reference's own scores are 0.17 off there -- so the bound is the
+    faithful error plus eps."""
+    import oracle
+
+    g = G.chung_lu(12000, 2.5, 10.0, seed=8)
+    lam, X = E.eigens_f32(*E.compute_eigens(g, 32))
+    R, _ = exact_resistance(g.idx, g.adj)
+    ferr = np.abs(oracle.tgtp(g.idx, g.adj, lam, X, 0.05, 10) - R)
+    for mode, name in MODES:
+        s, st = gpu_ctx.score(g, lam, X, 0.05, 10, mode=mode)
+        err = np.abs(s - R)
+        _say(f"  cl12000 {name}: max abs {err.max():.5f} mean {err.mean():.6f} "
+             f"(faithful {ferr.max():.5f} / {ferr.mean():.6f})")
+        assert err.max() <= ferr.max() + 0.05
+        assert err.mean() <= ferr.mean() + 0.01
+        assert np.mean(err > 0.05) <= np.mean(ferr > 0.05) + 0.001
+
+
+def test_fast_on_rewired_grid_tracks_faithful(gpu_ctx):
+    """Slow-mixing graph with long walks: the fast modes' errors against the
+    exact resistances stay within the faithful path's own error plus eps."""
+    g = G.grid_rewired(24, 24, 0.05, seed=2)
+    lam, X = E.eigens_f32(*E.compute_eigens(g, 32))
+    R, _ = exact_resistance(g.idx, g.adj)
+    f, _ = gpu_ctx.score(g, lam, X, 0.05, 10, mode=MODE_FAITHFUL)
+    ferr = np.abs(f - R)
+    for mode, name in MODES:
+        s, _ = gpu_ctx.score(g, lam, X, 0.05, 10, mode=mode)
+        err = np.abs(s - R)
+        _say(f"  grid24 {name}: max abs {err.max():.4f} (faithful {ferr.max():.4f}) "
+             f"mean {err.mean():.5f} (faithful {ferr.mean():.5f})")
+        assert err.max() <= ferr.max() + 0.05
+        assert err.mean() <= ferr.mean() + 0.025
