@@ -137,6 +137,34 @@ int tgtp_score(tgtp_ctx* ctx, const tgtp_graph* g, const tgtp_eigens* e,
 int tgtp_score_dev(tgtp_ctx* ctx, const tgtp_dgraph* dg, const tgtp_eigens* e,
                    const tgtp_params* p, float* scores_out, tgtp_stats* stats);
 
+/* ---- batched form -------------------------------------------------------------
+ * No counterpart in the reference, which scores one graph per process
+ * (benchmark.cpp:29-50).  Scores `count` graphs of uneven size together:
+ * in FAST modes they run as one disjoint-union CSR -- one tau projection
+ * launch, one persistent per-source launch over a work list spanning all
+ * graphs (largest first), one symmetrization -- optionally autotuned and
+ * captured into a HIP graph that every tgtp_batch_run replays.  All graphs
+ * share one mode; FAST batches need one omega in [3, 128].  FAITHFUL batches
+ * run the graphs one after another (bit-exact as tgtp_score).  Per-graph
+ * scores equal what tgtp_score_dev returns for that graph with the same
+ * tuning knobs.  scores_out[k] holds graphs[k].m floats. */
+typedef struct tgtp_batch tgtp_batch;
+#define TGTP_BATCH_AUTOTUNE 1 /* time candidate launch shapes once, keep the best */
+#define TGTP_BATCH_GRAPH 2    /* capture the run into a hipGraph, replay it */
+int tgtp_batch_create(tgtp_ctx* ctx, int32_t count, const tgtp_graph* graphs,
+                      const tgtp_eigens* eigens, const tgtp_params* params, int32_t flags,
+                      tgtp_batch** out);
+int tgtp_batch_run(tgtp_ctx* ctx, tgtp_batch* batch, float* const* scores_out,
+                   tgtp_stats* stats);
+void tgtp_batch_destroy(tgtp_ctx* ctx, tgtp_batch* batch);
+/* Launch shape the batch runs with (after autotuning, if requested). */
+int tgtp_batch_config(const tgtp_batch* batch, uint32_t* light_degree, uint32_t* small_bouquets,
+                      uint32_t* slots, uint32_t* max_slots);
+/* One-shot: create (no tuning, no capture), run, destroy. */
+int tgtp_score_batch(tgtp_ctx* ctx, int32_t count, const tgtp_graph* graphs,
+                     const tgtp_eigens* eigens, const tgtp_params* params,
+                     float* const* scores_out, tgtp_stats* stats);
+
 /* Diagnostic form of tgtp_score_dev that also returns the per-stage values
  * of the pipeline (each output pointer may be NULL): taus [m] (calculate_taus,
  * aesc.h:164-216), nr [m] walk count per position (0 where tau <= ell,

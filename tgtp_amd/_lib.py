@@ -24,6 +24,9 @@ MODE_FAITHFUL = 0
 MODE_FAST_FP32 = 1
 MODE_FAST_BF16 = 2
 
+BATCH_AUTOTUNE = 1
+BATCH_GRAPH = 2
+
 
 class TgtpError(RuntimeError):
     def __init__(self, code: int, message: str):
@@ -66,6 +69,8 @@ EXPORTED = [
     "tgtp_ctx_create", "tgtp_ctx_destroy", "tgtp_ctx_last_error", "tgtp_strerror",
     "tgtp_graph_upload", "tgtp_graph_free", "tgtp_saba_walks", "tgtp_taus",
     "tgtp_score", "tgtp_score_dev", "tgtp_score_trace",
+    "tgtp_batch_create", "tgtp_batch_run", "tgtp_batch_destroy", "tgtp_batch_config",
+    "tgtp_score_batch",
     "tgtp_load_txt", "tgtp_csr_free", "tgtp_load_eigens_txt", "tgtp_check_aesc_graph",
     "tgtp_gen_chung_lu", "tgtp_gen_grid_rewired", "tgtp_write_graph_txt",
     "tgtp_csr_reverse_index",
@@ -105,6 +110,15 @@ def lib() -> C.CDLL:
                                    P(StatsStruct), P(C.c_int32), P(C.c_uint64), P(C.c_float),
                                    P(C.c_float), P(C.c_int32), P(C.c_float), P(C.c_float),
                                    P(C.c_float), P(C.c_uint32)]
+    L.tgtp_batch_create.argtypes = [vp, C.c_int32, P(GraphStruct), P(EigensStruct),
+                                    P(ParamsStruct), C.c_int32, P(vp)]
+    L.tgtp_batch_run.argtypes = [vp, vp, P(P(C.c_float)), P(StatsStruct)]
+    L.tgtp_batch_destroy.argtypes = [vp, vp]
+    L.tgtp_batch_destroy.restype = None
+    L.tgtp_batch_config.argtypes = [vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32),
+                                    P(C.c_uint32)]
+    L.tgtp_score_batch.argtypes = [vp, C.c_int32, P(GraphStruct), P(EigensStruct),
+                                   P(ParamsStruct), P(P(C.c_float)), P(StatsStruct)]
     L.tgtp_load_txt.argtypes = [C.c_char_p, P(CsrStruct), C.c_char_p, C.c_size_t]
     L.tgtp_csr_free.argtypes = [P(CsrStruct)]
     L.tgtp_csr_free.restype = None

@@ -76,14 +76,13 @@ __global__ void __launch_bounds__(kTB) taus_fast_kernel(TauFastArgs A) {
   const uint32_t w = wave_id();
   const uint32_t r = l & 15u, kk = l >> 4;
   const uint32_t kq = A.omega_pad / 4;  // k values per lane group
-  const uint64_t n_tiles = (A.n_und + 15) / 16;
-  for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * (kTB / 64) + w; tile < n_tiles;
+  for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * (kTB / 64) + w; tile < A.n_tiles;
        tile += static_cast<uint64_t>(gridDim.x) * (kTB / 64)) {
-    const uint64_t e = tile * 16 + r;
-    const bool valid = e < A.n_und;
-    uint32_t p = 0, u = 0, v = 0;
+    const TauGraph G = A.graphs[A.tile_graph[tile]];
+    const uint32_t p = A.und[tile * 16 + r];
+    const bool valid = p != 0xFFFFFFFFu;
+    uint32_t u = 0, v = 0;
     if (valid) {
-      p = A.und[e];
       u = row_of_pos(A.idx, A.n, p);
       v = A.adj[p];
     }
@@ -123,23 +122,22 @@ __global__ void __launch_bounds__(kTB) taus_fast_kernel(TauFastArgs A) {
     double du = 0, dv = 0;
     float ups = 0.0f;
     int tau = 1, t = 1;
-    uint32_t k = 0;  // next refinement step
     bool done = !valid;
     if (l < 16 && valid) {
       du = static_cast<double>(A.idx[v + 1] - A.idx[v]);
       dv = static_cast<double>(A.idx[u + 1] - A.idx[u]);
     }
-    for (uint32_t ctile = 0; ctile < A.n_col_tiles; ++ctile) {
+    for (uint32_t ctile = 0; ctile < G.n_col_tiles; ++ctile) {
       f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-      const uint32_t col = ctile * 16 + r;
+      const uint64_t col = static_cast<uint64_t>(G.wt_row0) + ctile * 16 + r;
       if constexpr (!kBF16) {
-        const float* wcol = A.Wt + static_cast<uint64_t>(col) * A.omega_pad + k0;
+        const float* wcol = A.Wt + col * A.omega_pad + k0;
 #pragma unroll
         for (uint32_t s = 0; s < 32; ++s) {
           if (s < kq) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(dreg[s], wcol[s], acc, 0, 0, 0);
         }
       } else {
-        const __bf16* wcol = A.Wtb + static_cast<uint64_t>(col) * A.omega_pad + k0;
+        const __bf16* wcol = A.Wtb + col * A.omega_pad + k0;
 #pragma unroll
         for (uint32_t s = 0; s < 4; ++s) {
           if (8 * s < kq) {
@@ -158,17 +156,16 @@ __global__ void __launch_bounds__(kTB) taus_fast_kernel(TauFastArgs A) {
       if (l < 16 && !done) {
         uint32_t c = 0;
         if (ctile == 0) {
-          ups = ct[w][l][0] / static_cast<float>(A.m);
-          tau = tau_bound64(du, dv, A.eps, A.sq1m_1, A.b_1, 0.0f, 0.0f);
+          ups = ct[w][l][0] / G.m;
+          tau = tau_bound64(du, dv, G.eps, G.sq1m_1, G.b_1, 0.0f, 0.0f);
           c = 1;
         }
         for (; c < 16 && !done; ++c) {
-          const float del = ct[w][l][c] / static_cast<float>(A.m);
-          const int tp = tau_bound64(du, dv, A.eps, A.sq1m_w, A.b_w, del, ups);
+          const float del = ct[w][l][c] / G.m;
+          const int tp = tau_bound64(du, dv, G.eps, G.sq1m_w, G.b_w, del, ups);
           if (t <= tp && tp < tau) {
             tau = tp;
             t += 2;
-            ++k;
           } else {
             done = true;
           }
@@ -192,9 +189,8 @@ __global__ void to_bf16_kernel(const float* __restrict__ x, uint64_t count, __bf
 }  // namespace
 
 int launch_taus_fast(tgtp_ctx* ctx, const TauFastArgs& A, bool bf16) {
-  const uint64_t tiles = (A.n_und + 15) / 16;
   const uint64_t blocks = std::max<uint64_t>(
-      1, std::min<uint64_t>((tiles + 3) / 4, static_cast<uint64_t>(ctx->num_cus) * 8));
+      1, std::min<uint64_t>((A.n_tiles + 3) / 4, static_cast<uint64_t>(ctx->num_cus) * 8));
   if (bf16)
     taus_fast_kernel<true><<<static_cast<uint32_t>(blocks), kTB, 0, ctx->stream>>>(A);
   else

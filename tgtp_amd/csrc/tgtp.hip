@@ -34,6 +34,7 @@
 #include "common.hpp"
 #include "host_tgtp.hpp"
 #include "select.hpp"
+#include "tgtp_internal.hpp"
 #include "taus_fast.hpp"
 #include "tgtp_fast.hpp"
 
@@ -591,20 +592,60 @@ double now_ms() {
       .count();
 }
 
-// Device-side state attached to a dgraph for tgtp (host copies for fixups).
-struct TgtpPrep {
-  std::vector<uint32_t> h_idx, h_adj;
-  int32_t* d_src_of = nullptr;
-  uint32_t* d_und = nullptr;  // positions p with adj[p] >= row(p) (one per edge walk of calculate_taus)
-  uint64_t n_und = 0;
-  __bf16* d_xb = nullptr;     // BF16 copy of the cached eigenvectors
-  const void* xb_key = nullptr;
-};
+// Largest initial bound tau0 (aesc.h:188) per graph of a union CSR, plus 2
+// where ceil() is within rounding distance: sizes the refinement tables.
+__global__ void __launch_bounds__(256)
+tau0_per_graph_kernel(const uint32_t* __restrict__ idx, const uint32_t* __restrict__ adj,
+                      uint64_t m, uint32_t n, const uint32_t* __restrict__ voff, uint32_t n_graphs,
+                      const double* __restrict__ gp, uint32_t* g_tau0max) {
+  for (uint64_t p = blockIdx.x * 256ull + threadIdx.x; p < m; p += gridDim.x * 256ull) {
+    const uint32_t u = row_of(idx, n, p);
+    const uint32_t v = adj[p];
+    if (v < u) continue;
+    uint32_t lo = 0, hi = n_graphs;
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (voff[mid] <= u)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    bool near = false;
+    const double du = idx[v + 1] - idx[v], dv = idx[u + 1] - idx[u];
+    const int t0 = tau_bound_dev(du, dv, static_cast<float>(gp[3 * lo]), gp[3 * lo + 1],
+                                 gp[3 * lo + 2], 0.0f, 0.0f, near);
+    atomicMax(&g_tau0max[lo], static_cast<uint32_t>(t0) + (near ? 2u : 0u));
+  }
+}
 
+// Device-side state attached to a dgraph for tgtp (host copies for fixups).
 }  // namespace
 
+double tgtp_now_ms() { return now_ms(); }
+
+int launch_fill_i32(tgtp_ctx* ctx, int32_t* p, uint64_t n, int32_t v) {
+  fill_i32_kernel<<<grid_for(n, 256, 4096), 256, 0, ctx->stream>>>(p, n, v);
+  TGTP_HIP_CHECK(ctx, hipGetLastError());
+  return TGTP_OK;
+}
+
+int launch_fill_words(tgtp_ctx* ctx, uint32_t* p, uint32_t n) {
+  fill_words_kernel<<<1, 64, 0, ctx->stream>>>(p, n);
+  TGTP_HIP_CHECK(ctx, hipGetLastError());
+  return TGTP_OK;
+}
+
+int launch_tau0_per_graph(tgtp_ctx* ctx, const uint32_t* idx, const uint32_t* adj, uint64_t m,
+                          uint32_t n, const uint32_t* voff, uint32_t n_graphs, const double* gp,
+                          uint32_t* g_tau0max) {
+  tau0_per_graph_kernel<<<grid_for(m, 256, static_cast<uint32_t>(ctx->num_cus) * 16), 256, 0,
+                          ctx->stream>>>(idx, adj, m, n, voff, n_graphs, gp, g_tau0max);
+  TGTP_HIP_CHECK(ctx, hipGetLastError());
+  return TGTP_OK;
+}
+
 // Internal: fill host copies + symmetrization map once per dgraph.
-static int ensure_prep(tgtp_ctx* ctx, tgtp_dgraph* dg) {
+int tgtp_ensure_prep(tgtp_ctx* ctx, tgtp_dgraph* dg) {
   if (dg->prep) return TGTP_OK;
   auto* P = new (std::nothrow) TgtpPrep();
   if (!P) return set_error(ctx, TGTP_E_NOMEM, "host allocation");
@@ -624,31 +665,18 @@ static int ensure_prep(tgtp_ctx* ctx, tgtp_dgraph* dg) {
     return set_error(ctx, TGTP_E_NOMEM, "device allocation: symmetrization map");
   }
   TGTP_HIP_CHECK(ctx, hipMemcpyAsync(P->d_src_of, src.data(), dg->m * 4, hipMemcpyHostToDevice, s));
-  std::vector<uint32_t> und;
-  und.reserve(dg->m / 2 + 1);
-  for (uint64_t u = 0; u < dg->n; ++u)
-    for (uint64_t p = P->h_idx[u]; p < P->h_idx[u + 1]; ++p)
-      if (P->h_adj[p] >= u) und.push_back(static_cast<uint32_t>(p));
-  P->n_und = und.size();
-  if (hipMalloc(&P->d_und, std::max<uint64_t>(und.size(), 1) * 4) != hipSuccess) {
-    (void)hipFree(P->d_src_of);
-    delete P;
-    return set_error(ctx, TGTP_E_NOMEM, "device allocation: edge list");
-  }
-  TGTP_HIP_CHECK(ctx, hipMemcpyAsync(P->d_und, und.data(), und.size() * 4, hipMemcpyHostToDevice, s));
   TGTP_HIP_CHECK(ctx, hipStreamSynchronize(s));
   dg->prep = P;
   dg->prep_free = [](void* p) {
     auto* q = static_cast<TgtpPrep*>(p);
     if (q->d_src_of) (void)hipFree(q->d_src_of);
-    if (q->d_und) (void)hipFree(q->d_und);
-    if (q->d_xb) (void)hipFree(q->d_xb);
+    if (q->fast_plan && q->fast_plan_free) q->fast_plan_free(q->fast_plan);
     delete q;
   };
   return TGTP_OK;
 }
 
-static int ensure_eigens(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e) {
+int tgtp_ensure_eigens(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e) {
   if (!e || !e->values || !e->vectors || e->omega < 2 || static_cast<uint64_t>(e->omega) > dg->n)
     return set_error(ctx, TGTP_E_ARG, "omega must be between 2 and vertex count");
   if (dg->eig_key == e->vectors && dg->eig_omega == e->omega && dg->X) return TGTP_OK;
@@ -669,7 +697,7 @@ static int ensure_eigens(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e) {
 }
 
 // tau for every CSR position into d_taus (device) -- faithful arithmetic.
-static int run_taus_faithful(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e, float eps_half,
+int tgtp_run_taus_faithful(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e, float eps_half,
                              int32_t* d_taus, uint64_t* n_fixups) {
   hipStream_t s = ctx->stream;
   auto* P = static_cast<TgtpPrep*>(dg->prep);
@@ -750,103 +778,6 @@ static int run_taus_faithful(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* 
   return TGTP_OK;
 }
 
-// FAST modes: tau by MFMA projection (taus_fast.hip).  Eigenpair counts
-// above 128 use the faithful kernel.
-static int run_taus_fast(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e, float eps_half,
-                         int32_t mode, int32_t* d_taus, uint64_t* n_fixups) {
-  const int om = e->omega;
-  const char* force = std::getenv("TGTP_FAST_TAUS_FAITHFUL");
-  if (mode == TGTP_MODE_FAITHFUL || om > 128 || om < 3 || (force && *force == '1'))
-    return run_taus_faithful(ctx, dg, e, eps_half, d_taus, n_fixups);
-  *n_fixups = 0;
-  hipStream_t s = ctx->stream;
-  auto* P = static_cast<TgtpPrep*>(dg->prep);
-  const bool bf16 = mode == TGTP_MODE_FAST_BF16;
-  if (bf16 && (P->xb_key != dg->eig_key || !P->d_xb)) {
-    if (P->d_xb) (void)hipFree(P->d_xb);
-    P->d_xb = nullptr;
-    if (hipMalloc(&P->d_xb, dg->n * om * 2) != hipSuccess)
-      return set_error(ctx, TGTP_E_NOMEM, "device allocation: BF16 eigenvectors");
-    int rc = launch_to_bf16(ctx, dg->X, dg->n * om, P->d_xb);
-    if (rc) return rc;
-    P->xb_key = dg->eig_key;
-  }
-  // largest initial bound -> number of refinement columns needed
-  auto* d_words = static_cast<uint32_t*>(ctx_buffer(ctx, kBufFlags, 64));
-  if (!d_words) return set_error(ctx, TGTP_E_NOMEM, "tau scratch");
-  fill_words_kernel<<<1, 64, 0, s>>>(d_words, 16);
-  TauArgs A0{};
-  A0.idx = dg->idx;
-  A0.adj = dg->adj;
-  A0.m = dg->m;
-  A0.n = static_cast<uint32_t>(dg->n);
-  A0.eps = eps_half;
-  const double l1 = e->values[1], lw = e->values[om - 1];
-  A0.sq1m_1 = 1.0 - (l1 * l1);
-  A0.b_1 = std::log(1.0 / std::fabs(l1));
-  A0.tau0_max = d_words + 1;
-  const uint32_t cap = static_cast<uint32_t>(ctx->num_cus) * 16;
-  tau0_kernel<<<grid_for(dg->m, 256, cap), 256, 0, s>>>(A0);
-  TGTP_HIP_CHECK(ctx, hipGetLastError());
-  uint32_t words[2] = {0, 0};
-  TGTP_HIP_CHECK(ctx, hipMemcpyAsync(words, d_words, 8, hipMemcpyDeviceToHost, s));
-  TGTP_HIP_CHECK(ctx, hipStreamSynchronize(s));
-  const uint32_t ncols = 1 + std::min<uint32_t>(words[1] / 2 + 3, 1u << 16);
-  const uint32_t n_col_tiles = (ncols + 15) / 16;
-  const uint32_t omega_pad = static_cast<uint32_t>((om + 31) / 32 * 32);
-  const uint32_t rows = n_col_tiles * 16;
-  // W^T: row c = column c of W; c = 0: 1 + lambda_i; c >= 1: lambda_i^(2c) / (1 - lambda_i)
-  std::vector<double> pw = tgtp_host::power_table(e->values, om, rows);
-  std::vector<float> wt(static_cast<size_t>(rows) * omega_pad, 0.0f);
-  for (uint32_t c = 0; c < std::min(rows, ncols); ++c)
-    for (int i = 1; i < om - 1; ++i)
-      wt[static_cast<size_t>(c) * omega_pad + i] =
-          c == 0 ? static_cast<float>(1 + e->values[i])
-                 : static_cast<float>(pw[static_cast<size_t>(c - 1) * om + i] / (1 - e->values[i]));
-  auto* d_wt = static_cast<float*>(ctx_buffer(ctx, kBufEdgeInfo, wt.size() * 6 + 256));
-  if (!d_wt) return set_error(ctx, TGTP_E_NOMEM, "tau column table");
-  __bf16* d_wtb = reinterpret_cast<__bf16*>(d_wt + wt.size());
-  TGTP_HIP_CHECK(ctx, hipMemcpyAsync(d_wt, wt.data(), wt.size() * 4, hipMemcpyHostToDevice, s));
-  if (bf16) {
-    int rc = launch_to_bf16(ctx, d_wt, wt.size(), d_wtb);
-    if (rc) return rc;
-  }
-  fill_i32_kernel<<<grid_for(dg->m, 256, cap), 256, 0, s>>>(d_taus, dg->m, 0);
-  TauFastArgs A{};
-  A.idx = dg->idx;
-  A.adj = dg->adj;
-  A.und = P->d_und;
-  A.n_und = P->n_und;
-  A.X = dg->X;
-  A.Xb = P->d_xb;
-  A.Wt = d_wt;
-  A.Wtb = d_wtb;
-  A.m = dg->m;
-  A.n = static_cast<uint32_t>(dg->n);
-  A.omega = om;
-  A.omega_pad = omega_pad;
-  A.n_col_tiles = n_col_tiles;
-  A.eps = eps_half;
-  A.sq1m_1 = A0.sq1m_1;
-  A.b_1 = A0.b_1;
-  A.sq1m_w = 1.0 - (lw * lw);
-  A.b_w = std::log(1.0 / std::fabs(lw));
-  A.taus = d_taus;
-  return launch_taus_fast(ctx, A, bf16);
-}
-
-struct TraceOut {
-  int32_t* taus;
-  uint64_t* nr;
-  float* walk_sum;
-  float* preds_row;
-  int32_t* ell;
-  float* edge_max;
-  float* gmax;
-  float* gmin;
-  uint32_t* q_size;
-};
-
 static int run_score_faithful(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e,
                               const tgtp_params* prm, float* scores_out, tgtp_stats* stats,
                               const TraceOut* tr) {
@@ -861,7 +792,7 @@ static int run_score_faithful(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens*
   float* d_out = d_preds + m;
   TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[4], s));
   uint64_t fixups = 0;
-  int rc = run_taus_faithful(ctx, dg, e, static_cast<float>(prm->epsilon / 2.0), d_taus, &fixups);
+  int rc = tgtp_run_taus_faithful(ctx, dg, e, static_cast<float>(prm->epsilon / 2.0), d_taus, &fixups);
   if (rc) return rc;
   TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[5], s));
 
@@ -972,119 +903,13 @@ static int validate_params(tgtp_ctx* ctx, const tgtp_params* p) {
   return TGTP_OK;
 }
 
-// FAST_FP32 / FAST_BF16: tau bounds, synchronous per-source pipeline with
-// FP64 row sums (tgtp_fast.hip), FP64 symmetrization.
-static int run_score_fast(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e,
-                          const tgtp_params* prm, float* scores_out, tgtp_stats* stats,
-                          const TraceOut* tr) {
-  const double t_host0 = now_ms();
-  hipStream_t s = ctx->stream;
-  const uint64_t n = dg->n, m = dg->m;
-  auto* P = static_cast<TgtpPrep*>(dg->prep);
-  auto* d_taus = static_cast<int32_t*>(ctx_buffer(ctx, kBufTaus, std::max<uint64_t>(m, 1) * 4));
-  auto* d_preds = static_cast<double*>(ctx_buffer(ctx, kBufPreds, std::max<uint64_t>(m, 1) * 12 + 16));
-  auto* d_nr = static_cast<uint64_t*>(ctx_buffer(ctx, kBufSrcInfo, std::max<uint64_t>(m, 1) * 8));
-  if (!d_taus || !d_preds || !d_nr) return set_error(ctx, TGTP_E_NOMEM, "tgtp buffers");
-  float* d_out = reinterpret_cast<float*>(d_preds + m);
-  TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[4], s));
-  uint64_t fixups = 0;
-  int rc = run_taus_fast(ctx, dg, e, static_cast<float>(prm->epsilon / 2.0), prm->mode, d_taus,
-                         &fixups);
-  if (rc) return rc;
-  TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[5], s));
-  const bool use_lds = fast_fits_lds(n);
-  const int per_cu = fast_slots_per_cu(use_lds, n);
-  const uint32_t slots =
-      static_cast<uint32_t>(std::min<uint64_t>(n, static_cast<uint64_t>(ctx->num_cus) * per_cu));
-  auto* d_q = static_cast<uint32_t*>(ctx_buffer(ctx, kBufQueue, static_cast<uint64_t>(slots) * 2 * n * 4));
-  auto* d_words = static_cast<uint32_t*>(ctx_buffer(ctx, kBufSums, 64));
-  float* d_pv = nullptr;
-  if (!use_lds) {
-    d_pv = static_cast<float*>(
-        ctx_buffer(ctx, kBufPvec, static_cast<uint64_t>(slots) * (2 * n + (n + 31) / 32) * 4));
-    if (!d_pv) return set_error(ctx, TGTP_E_NOMEM, "per-source state");
-  }
-  if (!d_q || !d_words) return set_error(ctx, TGTP_E_NOMEM, "frontier queues");
-  fill_words_kernel<<<1, 64, 0, s>>>(d_words, 16);
-  FastArgs A{};
-  A.rowd = dg->rowd;
-  A.adj = dg->adj;
-  A.degf = dg->degf;
-  A.taus = d_taus;
-  A.preds_d = d_preds;
-  A.nr_pos = d_nr;
-  A.qbuf = d_q;
-  A.pv_g = d_pv;
-  A.next_src = d_words;
-  A.status = d_words + 2;
-  A.n = static_cast<uint32_t>(n);
-  A.gamma = prm->gamma;
-  A.eps = prm->epsilon;
-  A.logmd = std::log(m / prm->delta);
-  float* tr_buf = nullptr;
-  if (tr) {
-    tr_buf = static_cast<float*>(ctx_buffer(ctx, kBufTrace, (5 * n + m) * 4 + 64));
-    if (!tr_buf) return set_error(ctx, TGTP_E_NOMEM, "trace buffers");
-    A.tr_ell = reinterpret_cast<int32_t*>(tr_buf);
-    A.tr_emax = tr_buf + n;
-    A.tr_gmax = tr_buf + 2 * n;
-    A.tr_gmin = tr_buf + 3 * n;
-    A.tr_qsize = reinterpret_cast<uint32_t*>(tr_buf + 4 * n);
-    A.tr_x = tr_buf + 5 * n;
-    fill_i32_kernel<<<grid_for(5 * n + m, 256, 4096), 256, 0, s>>>(
-        reinterpret_cast<int32_t*>(tr_buf), 5 * n + m, 0);
-  }
-  if ((rc = launch_fast_sources(ctx, A, slots, use_lds))) return rc;
-  TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[6], s));
-  if ((rc = launch_symmetrize_d(ctx, d_preds, P->d_src_of, m, d_out))) return rc;
-  TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[7], s));
-  TGTP_HIP_CHECK(ctx, hipMemcpyAsync(scores_out, d_out, m * 4, hipMemcpyDeviceToHost, s));
-  uint32_t words[3] = {0, 0, 0};
-  TGTP_HIP_CHECK(ctx, hipMemcpyAsync(words, d_words, 12, hipMemcpyDeviceToHost, s));
-  if (tr) {
-    if (tr->taus) TGTP_HIP_CHECK(ctx, hipMemcpyAsync(tr->taus, d_taus, m * 4, hipMemcpyDeviceToHost, s));
-    if (tr->nr) TGTP_HIP_CHECK(ctx, hipMemcpyAsync(tr->nr, d_nr, m * 8, hipMemcpyDeviceToHost, s));
-    if (tr->walk_sum)
-      TGTP_HIP_CHECK(ctx, hipMemcpyAsync(tr->walk_sum, A.tr_x, m * 4, hipMemcpyDeviceToHost, s));
-    if (tr->ell) TGTP_HIP_CHECK(ctx, hipMemcpyAsync(tr->ell, A.tr_ell, n * 4, hipMemcpyDeviceToHost, s));
-    if (tr->edge_max)
-      TGTP_HIP_CHECK(ctx, hipMemcpyAsync(tr->edge_max, A.tr_emax, n * 4, hipMemcpyDeviceToHost, s));
-    if (tr->gmax) TGTP_HIP_CHECK(ctx, hipMemcpyAsync(tr->gmax, A.tr_gmax, n * 4, hipMemcpyDeviceToHost, s));
-    if (tr->gmin) TGTP_HIP_CHECK(ctx, hipMemcpyAsync(tr->gmin, A.tr_gmin, n * 4, hipMemcpyDeviceToHost, s));
-    if (tr->q_size)
-      TGTP_HIP_CHECK(ctx, hipMemcpyAsync(tr->q_size, A.tr_qsize, n * 4, hipMemcpyDeviceToHost, s));
-  }
-  TGTP_HIP_CHECK(ctx, hipStreamSynchronize(s));
-  if (tr && tr->preds_row) {
-    std::vector<double> pd(m);
-    TGTP_HIP_CHECK(ctx, hipMemcpy(pd.data(), d_preds, m * 8, hipMemcpyDeviceToHost));
-    for (uint64_t i = 0; i < m; ++i) tr->preds_row[i] = static_cast<float>(pd[i]);
-  }
-  if (words[2] & 1u)
-    return set_error(ctx, TGTP_E_UNSUPPORTED,
-                     "a walk count exceeded 2^36 walks per edge; scores are not computed");
-  if (stats) {
-    std::memset(stats, 0, sizeof(*stats));
-    float a = 0, b = 0, c = 0;
-    TGTP_HIP_CHECK(ctx, hipEventElapsedTime(&a, ctx->ev[4], ctx->ev[7]));
-    TGTP_HIP_CHECK(ctx, hipEventElapsedTime(&b, ctx->ev[4], ctx->ev[5]));
-    TGTP_HIP_CHECK(ctx, hipEventElapsedTime(&c, ctx->ev[5], ctx->ev[6]));
-    stats->kernel_ms = a;
-    stats->taus_ms = b;
-    stats->diffuse_ms = c;
-    stats->host_fixups = fixups;
-    stats->e2e_ms = now_ms() - t_host0;
-  }
-  return TGTP_OK;
-}
-
 static int run_score(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e, const tgtp_params* p,
                      float* out, tgtp_stats* stats, const TraceOut* tr) {
   if (p->mode == TGTP_MODE_FAITHFUL) return run_score_faithful(ctx, dg, e, p, out, stats, tr);
-  return run_score_fast(ctx, dg, e, p, out, stats, tr);
+  return tgtp_run_score_fast(ctx, dg, e, p, out, stats, tr);
 }
 
-static int check_degrees(tgtp_ctx* ctx, tgtp_dgraph* dg) {
+int tgtp_check_degrees(tgtp_ctx* ctx, tgtp_dgraph* dg) {
   auto* P = static_cast<TgtpPrep*>(dg->prep);
   for (uint64_t v = 0; v < dg->n; ++v)
     if (P->h_idx[v + 1] == P->h_idx[v])
@@ -1101,9 +926,9 @@ extern "C" int tgtp_score_dev(tgtp_ctx* ctx, const tgtp_dgraph* cdg, const tgtp_
   int rc = validate_params(ctx, p);
   if (rc) return rc;
   TGTP_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-  if ((rc = ensure_prep(ctx, dg))) return rc;
-  if ((rc = check_degrees(ctx, dg))) return rc;
-  if ((rc = ensure_eigens(ctx, dg, e))) return rc;
+  if ((rc = tgtp_ensure_prep(ctx, dg))) return rc;
+  if ((rc = tgtp_check_degrees(ctx, dg))) return rc;
+  if ((rc = tgtp_ensure_eigens(ctx, dg, e))) return rc;
   return run_score(ctx, dg, e, p, scores_out, stats, nullptr);
 }
 
@@ -1131,9 +956,9 @@ extern "C" int tgtp_score_trace(tgtp_ctx* ctx, const tgtp_dgraph* cdg, const tgt
   int rc = validate_params(ctx, p);
   if (rc) return rc;
   TGTP_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-  if ((rc = ensure_prep(ctx, dg))) return rc;
-  if ((rc = check_degrees(ctx, dg))) return rc;
-  if ((rc = ensure_eigens(ctx, dg, e))) return rc;
+  if ((rc = tgtp_ensure_prep(ctx, dg))) return rc;
+  if ((rc = tgtp_check_degrees(ctx, dg))) return rc;
+  if ((rc = tgtp_ensure_eigens(ctx, dg, e))) return rc;
   TraceOut tr{taus, nr, walk_sum, preds_row, ell, edge_max, gmax, gmin, q_size};
   return run_score(ctx, dg, e, p, scores_out, stats, &tr);
 }
@@ -1153,14 +978,15 @@ extern "C" int tgtp_taus(tgtp_ctx* ctx, const tgtp_graph* g, const tgtp_eigens* 
     tgtp_graph_free(ctx, dg);
     return r;
   };
-  if ((rc = ensure_prep(ctx, dg))) return cleanup(rc);
-  if ((rc = ensure_eigens(ctx, dg, e))) return cleanup(rc);
+  if ((rc = tgtp_ensure_prep(ctx, dg))) return cleanup(rc);
+  if ((rc = tgtp_ensure_eigens(ctx, dg, e))) return cleanup(rc);
   auto* d_taus = static_cast<int32_t*>(ctx_buffer(ctx, kBufTaus, std::max<uint64_t>(dg->m, 1) * 4));
   if (!d_taus) return cleanup(set_error(ctx, TGTP_E_NOMEM, "taus buffer"));
   hipStream_t s = ctx->stream;
   if (hipEventRecord(ctx->ev[4], s) != hipSuccess) return cleanup(TGTP_E_HIP);
   uint64_t fixups = 0;
-  rc = run_taus_fast(ctx, dg, e, eps_half, mode, d_taus, &fixups);
+  rc = mode == TGTP_MODE_FAITHFUL ? tgtp_run_taus_faithful(ctx, dg, e, eps_half, d_taus, &fixups)
+                                 : tgtp_run_taus_fast(ctx, dg, e, eps_half, mode, d_taus);
   if (rc) return cleanup(rc);
   if (hipEventRecord(ctx->ev[5], s) != hipSuccess ||
       hipMemcpyAsync(taus_out, d_taus, dg->m * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||

@@ -1,0 +1,700 @@
+// tgtp_batch.hip -- FAST_FP32 / FAST_BF16 scoring of one graph or of a batch
+// of graphs, and the batched C ABI (tgtp_batch_*, tgtp_score_batch).
+//
+// A batch runs as the disjoint union of its graphs: one CSR (vertex ids
+// shifted per graph), one eigenvector array, per-graph tables.  A FAST plan
+// holds everything a run needs on the device -- per-graph scalars, the
+// padded 16-edge tiles of the tau projection with their column tables, the
+// work list of sources (largest graphs first, so the longest sources start
+// first) and all scratch -- so a run is four kernels on one stream:
+// clear, tau projection, per-source pipeline, symmetrization.  The batch
+// captures that sequence into a hipGraph once and replays it.
+//
+// Autotuning (TGTP_BATCH_AUTOTUNE): coordinate descent over the light-vertex
+// degree threshold of the frontier kernels, the bouquet count below which a
+// walk position gets one 8-lane group instead of the whole workgroup, and the
+// number of resident workgroups, each candidate timed with HIP events on a
+// full run; the best is kept for the batch's captured graph.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "common.hpp"
+#include "host_tgtp.hpp"
+#include "taus_fast.hpp"
+#include "tgtp_fast.hpp"
+#include "tgtp_internal.hpp"
+
+namespace {
+
+struct GraphSpec {        // host description of one graph of a plan
+  uint32_t voff, n;       // vertex range in the union
+  uint64_t eoff, m;       // CSR range in the union
+  const double* lam;      // [omega] host eigenvalues
+  double eps, delta;
+  int32_t gamma;
+};
+
+struct FastPlan {
+  // identity (a plan cached on a single dgraph is reused when these match)
+  const void* eig_key = nullptr;
+  double eps = 0, delta = 0;
+  int32_t gamma = 0, mode = 0;
+  uint32_t n_graphs = 0;
+  uint64_t n = 0, m = 0;
+  int32_t omega = 0;
+  uint32_t omega_pad = 0;
+  // device tables
+  uint32_t* d_voff = nullptr;
+  double* d_eps = nullptr;
+  double* d_logmd = nullptr;
+  int32_t* d_gamma = nullptr;
+  TauGraph* d_tgraphs = nullptr;
+  uint32_t* d_und = nullptr;
+  uint32_t* d_tile_graph = nullptr;
+  uint64_t n_tiles = 0;
+  float* d_wt = nullptr;
+  __bf16* d_wtb = nullptr;
+  __bf16* d_xb = nullptr;
+  uint32_t* d_work = nullptr;
+  uint32_t n_work = 0;
+  // scratch
+  int32_t* d_taus = nullptr;
+  double* d_preds = nullptr;
+  uint64_t* d_nr = nullptr;
+  uint32_t* d_q = nullptr;
+  float* d_pv = nullptr;
+  uint32_t* d_words = nullptr;
+  float* d_out = nullptr;
+  uint32_t lds_n = 0, slot_n = 0, qcap = 0;
+  uint32_t max_slots = 0, slots = 0;
+  // tuned parameters
+  uint32_t light = 32, small_bq = 8;
+  // captured run
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+void free_plan(FastPlan* P) {
+  if (!P) return;
+  if (P->exec) (void)hipGraphExecDestroy(P->exec);
+  if (P->graph) (void)hipGraphDestroy(P->graph);
+  void* ptrs[] = {P->d_voff, P->d_eps, P->d_logmd, P->d_gamma, P->d_tgraphs, P->d_und,
+                  P->d_tile_graph, P->d_wt, P->d_wtb, P->d_xb, P->d_work, P->d_taus,
+                  P->d_preds, P->d_nr, P->d_q, P->d_pv, P->d_words, P->d_out};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete P;
+}
+
+template <typename T>
+int dev_alloc(tgtp_ctx* ctx, T** p, uint64_t count, const char* what) {
+  if (hipMalloc(reinterpret_cast<void**>(p), std::max<uint64_t>(count, 1) * sizeof(T)) !=
+      hipSuccess) {
+    *p = nullptr;
+    return set_error(ctx, TGTP_E_NOMEM, std::string("device allocation: ") + what);
+  }
+  return TGTP_OK;
+}
+
+template <typename T>
+int dev_upload(tgtp_ctx* ctx, T** p, const std::vector<T>& host, const char* what) {
+  int rc = dev_alloc(ctx, p, host.size(), what);
+  if (rc) return rc;
+  if (!host.empty())
+    TGTP_HIP_CHECK(ctx, hipMemcpyAsync(*p, host.data(), host.size() * sizeof(T),
+                                       hipMemcpyHostToDevice, ctx->stream));
+  return TGTP_OK;
+}
+
+// Builds the device tables and scratch of a FAST run over `dg` (a single
+// graph or a union) whose eigenvectors are already in dg->X.
+int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& specs, int32_t omega,
+               int32_t mode, FastPlan** out) {
+  auto* prep = static_cast<TgtpPrep*>(dg->prep);
+  auto* P = new (std::nothrow) FastPlan();
+  if (!P) return set_error(ctx, TGTP_E_NOMEM, "host allocation");
+  auto fail = [&](int rc) {
+    free_plan(P);
+    return rc;
+  };
+  hipStream_t s = ctx->stream;
+  const uint32_t G = static_cast<uint32_t>(specs.size());
+  P->n_graphs = G;
+  P->n = dg->n;
+  P->m = dg->m;
+  P->mode = mode;
+  P->omega = omega;
+  P->omega_pad = static_cast<uint32_t>((omega + 31) / 32 * 32);
+  if (G == 1) {
+    P->eps = specs[0].eps;
+    P->delta = specs[0].delta;
+    P->gamma = specs[0].gamma;
+    P->eig_key = dg->eig_key;
+  }
+  // per-graph scalars
+  std::vector<uint32_t> voff(G + 1);
+  std::vector<double> g_eps(G), g_logmd(G), tau0p(3 * G);
+  std::vector<int32_t> g_gamma(G);
+  for (uint32_t k = 0; k < G; ++k) {
+    const GraphSpec& S = specs[k];
+    voff[k] = S.voff;
+    g_eps[k] = S.eps;
+    g_logmd[k] = std::log(S.m / S.delta);  // log(g.m / config.delta), aesc.h:498,520
+    g_gamma[k] = S.gamma;
+    const double l1 = S.lam[1];
+    tau0p[3 * k] = static_cast<double>(static_cast<float>(S.eps / 2.0));
+    tau0p[3 * k + 1] = 1.0 - (l1 * l1);
+    tau0p[3 * k + 2] = std::log(1.0 / std::fabs(l1));
+  }
+  voff[G] = static_cast<uint32_t>(dg->n);
+  int rc;
+  double* d_tau0p = nullptr;
+  uint32_t* d_tau0max = nullptr;
+  if ((rc = dev_upload(ctx, &P->d_voff, voff, "graph offsets")) ||
+      (rc = dev_upload(ctx, &P->d_eps, g_eps, "graph eps")) ||
+      (rc = dev_upload(ctx, &P->d_logmd, g_logmd, "graph log(m/delta)")) ||
+      (rc = dev_upload(ctx, &P->d_gamma, g_gamma, "graph gamma")) ||
+      (rc = dev_upload(ctx, &d_tau0p, tau0p, "tau0 params")) ||
+      (rc = dev_upload(ctx, &d_tau0max, std::vector<uint32_t>(G, 0u), "tau0 max")))
+    return fail(rc);
+  rc = launch_tau0_per_graph(ctx, dg->idx, dg->adj, dg->m, static_cast<uint32_t>(dg->n), P->d_voff,
+                             G, d_tau0p, d_tau0max);
+  std::vector<uint32_t> tau0max(G);
+  if (!rc &&
+      hipMemcpyAsync(tau0max.data(), d_tau0max, G * 4, hipMemcpyDeviceToHost, s) == hipSuccess &&
+      hipStreamSynchronize(s) == hipSuccess) {
+  } else if (!rc) {
+    rc = set_error(ctx, TGTP_E_HIP, "tau0 readback");
+  }
+  (void)hipFree(d_tau0p);
+  (void)hipFree(d_tau0max);
+  if (rc) return fail(rc);
+  // column tables: row c of graph k = column c of W_k
+  std::vector<TauGraph> tg(G);
+  uint32_t rows_total = 0;
+  for (uint32_t k = 0; k < G; ++k) {
+    const uint32_t ncols = 1 + std::min<uint32_t>(tau0max[k] / 2 + 3, 1u << 16);
+    tg[k].n_col_tiles = (ncols + 15) / 16;
+    tg[k].wt_row0 = rows_total;
+    rows_total += tg[k].n_col_tiles * 16;
+  }
+  std::vector<float> wt(static_cast<size_t>(rows_total) * P->omega_pad, 0.0f);
+  for (uint32_t k = 0; k < G; ++k) {
+    const GraphSpec& S = specs[k];
+    const uint32_t rows = tg[k].n_col_tiles * 16;
+    const std::vector<double> pw = tgtp_host::power_table(S.lam, omega, rows);
+    for (uint32_t c = 0; c < rows; ++c) {
+      float* row = wt.data() + static_cast<size_t>(tg[k].wt_row0 + c) * P->omega_pad;
+      for (int i = 1; i < omega - 1; ++i)
+        row[i] = c == 0 ? static_cast<float>(1 + S.lam[i])
+                        : static_cast<float>(pw[static_cast<size_t>(c - 1) * omega + i] /
+                                             (1 - S.lam[i]));
+    }
+    const double lw = S.lam[omega - 1];
+    tg[k].sq1m_1 = tau0p[3 * k + 1];
+    tg[k].b_1 = tau0p[3 * k + 2];
+    tg[k].sq1m_w = 1.0 - (lw * lw);
+    tg[k].b_w = std::log(1.0 / std::fabs(lw));
+    tg[k].m = static_cast<float>(S.m);
+    tg[k].eps = static_cast<float>(S.eps / 2.0);
+  }
+  if ((rc = dev_upload(ctx, &P->d_wt, wt, "tau column tables")) ||
+      (rc = dev_upload(ctx, &P->d_tgraphs, tg, "tau graph params")))
+    return fail(rc);
+  if (mode == TGTP_MODE_FAST_BF16) {
+    if ((rc = dev_alloc(ctx, &P->d_wtb, wt.size(), "bf16 tables")) ||
+        (rc = launch_to_bf16(ctx, P->d_wt, wt.size(), P->d_wtb)) ||
+        (rc = dev_alloc(ctx, &P->d_xb, dg->n * omega, "bf16 eigenvectors")) ||
+        (rc = launch_to_bf16(ctx, dg->X, dg->n * omega, P->d_xb)))
+      return fail(rc);
+  }
+  // 16-edge tiles per graph: positions p with adj[p] >= row(p), padded
+  std::vector<uint32_t> und, tile_graph;
+  und.reserve(dg->m / 2 + 16 * G);
+  const uint32_t* idx = prep->h_idx.data();
+  const uint32_t* adj = prep->h_adj.data();
+  for (uint32_t k = 0; k < G; ++k) {
+    const GraphSpec& S = specs[k];
+    const size_t start = und.size();
+    for (uint64_t u = S.voff; u < S.voff + S.n; ++u)
+      for (uint64_t p = idx[u]; p < idx[u + 1]; ++p)
+        if (adj[p] >= u) und.push_back(static_cast<uint32_t>(p));
+    while ((und.size() - start) % 16) und.push_back(0xFFFFFFFFu);
+    tile_graph.insert(tile_graph.end(), (und.size() - start) / 16, k);
+  }
+  P->n_tiles = tile_graph.size();
+  if ((rc = dev_upload(ctx, &P->d_und, und, "tau tiles")) ||
+      (rc = dev_upload(ctx, &P->d_tile_graph, tile_graph, "tile graphs")))
+    return fail(rc);
+  // work list: sources of the largest graphs first
+  std::vector<uint32_t> order(G);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t a, uint32_t b) { return specs[a].m > specs[b].m; });
+  std::vector<uint32_t> work;
+  work.reserve(dg->n);
+  for (uint32_t k : order)
+    for (uint32_t v = 0; v < specs[k].n; ++v) work.push_back(specs[k].voff + v);
+  P->n_work = static_cast<uint32_t>(work.size());
+  if ((rc = dev_upload(ctx, &P->d_work, work, "work list"))) return fail(rc);
+  // per-source state: LDS for graphs that fit, HBM slots for the rest
+  for (const GraphSpec& S : specs) {
+    P->qcap = std::max(P->qcap, S.n);
+    if (fast_fits_lds(S.n))
+      P->lds_n = std::max(P->lds_n, S.n);
+    else
+      P->slot_n = std::max(P->slot_n, S.n);
+  }
+  const int per_cu = fast_slots_per_cu(P->lds_n);
+  P->max_slots = static_cast<uint32_t>(
+      std::min<uint64_t>(P->n_work, static_cast<uint64_t>(ctx->num_cus) * per_cu));
+  P->slots = P->max_slots;
+  if ((rc = dev_alloc(ctx, &P->d_q, static_cast<uint64_t>(P->max_slots) * 2 * P->qcap, "queues")))
+    return fail(rc);
+  if (P->slot_n &&
+      (rc = dev_alloc(ctx, &P->d_pv, P->max_slots * fast_slot_floats(P->slot_n), "source state")))
+    return fail(rc);
+  if ((rc = dev_alloc(ctx, &P->d_taus, dg->m, "taus")) ||
+      (rc = dev_alloc(ctx, &P->d_preds, dg->m, "preds")) ||
+      (rc = dev_alloc(ctx, &P->d_nr, dg->m, "walk counts")) ||
+      (rc = dev_alloc(ctx, &P->d_out, dg->m, "scores")) ||
+      (rc = dev_alloc(ctx, &P->d_words, 16, "counters")))
+    return fail(rc);
+  TGTP_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  *out = P;
+  return TGTP_OK;
+}
+
+FastArgs plan_args(const FastPlan* P, tgtp_dgraph* dg) {
+  FastArgs A{};
+  A.rowd = dg->rowd;
+  A.adj = dg->adj;
+  A.degf = dg->degf;
+  A.taus = P->d_taus;
+  A.preds_d = P->d_preds;
+  A.nr_pos = P->d_nr;
+  A.qbuf = P->d_q;
+  A.pv_g = P->d_pv;
+  A.next_src = P->d_words;
+  A.status = P->d_words + 2;
+  A.voff = P->d_voff;
+  A.n_graphs = P->n_graphs;
+  A.g_eps = P->d_eps;
+  A.g_logmd = P->d_logmd;
+  A.g_gamma = P->d_gamma;
+  A.work = P->d_work;
+  A.n_work = P->n_work;
+  A.lds_n = P->lds_n;
+  A.slot_n = P->slot_n;
+  A.qcap = P->qcap;
+  A.light = P->light;
+  A.small_bq = P->small_bq;
+  return A;
+}
+
+TauFastArgs plan_tau_args(const FastPlan* P, tgtp_dgraph* dg) {
+  TauFastArgs T{};
+  T.idx = dg->idx;
+  T.adj = dg->adj;
+  T.und = P->d_und;
+  T.tile_graph = P->d_tile_graph;
+  T.graphs = P->d_tgraphs;
+  T.n_tiles = P->n_tiles;
+  T.X = dg->X;
+  T.Xb = P->d_xb;
+  T.Wt = P->d_wt;
+  T.Wtb = P->d_wtb;
+  T.n = static_cast<uint32_t>(dg->n);
+  T.omega = P->omega;
+  T.omega_pad = P->omega_pad;
+  T.taus = P->d_taus;
+  return T;
+}
+
+// The run itself: device work only (capturable).  Events 5 and 6 (when
+// record_events) split tau / per-source / symmetrization.
+int enqueue_run(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_of,
+                bool record_events, const FastArgs* override_args) {
+  int rc;
+  if ((rc = launch_fill_words(ctx, P->d_words, 16)) ||
+      (rc = launch_fill_i32(ctx, P->d_taus, P->m, 0)) ||
+      (rc = launch_taus_fast(ctx, plan_tau_args(P, dg), P->mode == TGTP_MODE_FAST_BF16)))
+    return rc;
+  if (record_events) TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+  const FastArgs A = override_args ? *override_args : plan_args(P, dg);
+  if ((rc = launch_fast_sources(ctx, A, P->slots))) return rc;
+  if (record_events) TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
+  return launch_symmetrize_d(ctx, P->d_preds, src_of, P->m, P->d_out);
+}
+
+int timed_run(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_of, float* ms) {
+  TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+  int rc = enqueue_run(ctx, P, dg, src_of, false, nullptr);
+  if (rc) return rc;
+  TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[7], ctx->stream));
+  TGTP_HIP_CHECK(ctx, hipEventSynchronize(ctx->ev[7]));
+  TGTP_HIP_CHECK(ctx, hipEventElapsedTime(ms, ctx->ev[4], ctx->ev[7]));
+  return TGTP_OK;
+}
+
+int autotune(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_of) {
+  float best = 0;
+  int rc = timed_run(ctx, P, dg, src_of, &best);
+  if (rc) return rc;
+  auto trial = [&](uint32_t* knob, std::initializer_list<uint32_t> values) -> int {
+    const uint32_t keep = *knob;
+    uint32_t pick = keep;
+    for (uint32_t v : values) {
+      if (v == keep) continue;
+      *knob = v;
+      float ms = 0;
+      int r = timed_run(ctx, P, dg, src_of, &ms);
+      if (r) return r;
+      if (ms < best) {
+        best = ms;
+        pick = v;
+      }
+    }
+    *knob = pick;
+    return TGTP_OK;
+  };
+  if ((rc = trial(&P->light, {16, 32, 64}))) return rc;
+  if ((rc = trial(&P->small_bq, {2, 8, 32}))) return rc;
+  const uint32_t full = P->max_slots;
+  if ((rc = trial(&P->slots, {full, std::max(1u, full / 2), std::max(1u, (3 * full) / 4)})))
+    return rc;
+  return TGTP_OK;
+}
+
+int capture(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_of) {
+  TGTP_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_run(ctx, P, dg, src_of, false, nullptr);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return set_error(ctx, TGTP_E_HIP, "hipStreamEndCapture failed");
+  P->graph = g;
+  if (hipGraphInstantiate(&P->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+    P->exec = nullptr;
+    return set_error(ctx, TGTP_E_HIP, "hipGraphInstantiate failed");
+  }
+  return TGTP_OK;
+}
+
+int check_status(tgtp_ctx* ctx, FastPlan* P) {
+  uint32_t words[3] = {0, 0, 0};
+  TGTP_HIP_CHECK(ctx, hipMemcpy(words, P->d_words, 12, hipMemcpyDeviceToHost));
+  if (words[2] & 1u)
+    return set_error(ctx, TGTP_E_UNSUPPORTED,
+                     "a walk count exceeded 2^36 walks per edge; scores are not computed");
+  return TGTP_OK;
+}
+
+bool plan_matches(const FastPlan* P, const tgtp_dgraph* dg, const tgtp_params* p) {
+  return P && P->eig_key == dg->eig_key && P->eps == p->epsilon && P->delta == p->delta &&
+         P->gamma == p->gamma && P->mode == p->mode && P->omega == dg->eig_omega;
+}
+
+int single_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e, const tgtp_params* p,
+                FastPlan** out) {
+  auto* prep = static_cast<TgtpPrep*>(dg->prep);
+  auto* cached = static_cast<FastPlan*>(prep->fast_plan);
+  if (plan_matches(cached, dg, p)) {
+    *out = cached;
+    return TGTP_OK;
+  }
+  if (cached) free_plan(cached);
+  prep->fast_plan = nullptr;
+  std::vector<GraphSpec> specs(1);
+  specs[0] = GraphSpec{0u, static_cast<uint32_t>(dg->n), 0ull, dg->m, e->values, p->epsilon,
+                       p->delta, p->gamma};
+  FastPlan* P = nullptr;
+  const int rc = build_plan(ctx, dg, specs, e->omega, p->mode, &P);
+  if (rc) return rc;
+  prep->fast_plan = P;
+  prep->fast_plan_free = [](void* q) { free_plan(static_cast<FastPlan*>(q)); };
+  *out = P;
+  return TGTP_OK;
+}
+
+}  // namespace
+
+// ---- single graph -------------------------------------------------------------
+
+int tgtp_run_score_fast(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e,
+                        const tgtp_params* p, float* scores_out, tgtp_stats* stats,
+                        const TraceOut* tr) {
+  const double t0 = tgtp_now_ms();
+  if (e->omega > 128 || e->omega < 3)
+    return set_error(ctx, TGTP_E_UNSUPPORTED, "fast modes need 3 <= omega <= 128");
+  auto* prep = static_cast<TgtpPrep*>(dg->prep);
+  FastPlan* P = nullptr;
+  int rc = single_plan(ctx, dg, e, p, &P);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  float* tr_buf = nullptr;
+  FastArgs A = plan_args(P, dg);
+  if (tr) {
+    const uint64_t n = dg->n, m = dg->m;
+    tr_buf = static_cast<float*>(ctx_buffer(ctx, kBufTrace, (5 * n + m) * 4 + 64));
+    if (!tr_buf) return set_error(ctx, TGTP_E_NOMEM, "trace buffers");
+    A.tr_ell = reinterpret_cast<int32_t*>(tr_buf);
+    A.tr_emax = tr_buf + n;
+    A.tr_gmax = tr_buf + 2 * n;
+    A.tr_gmin = tr_buf + 3 * n;
+    A.tr_qsize = reinterpret_cast<uint32_t*>(tr_buf + 4 * n);
+    A.tr_x = tr_buf + 5 * n;
+    if ((rc = launch_fill_i32(ctx, reinterpret_cast<int32_t*>(tr_buf), 5 * n + m, 0))) return rc;
+  }
+  TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[4], s));
+  if ((rc = enqueue_run(ctx, P, dg, prep->d_src_of, true, &A))) return rc;
+  TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[7], s));
+  TGTP_HIP_CHECK(ctx, hipMemcpyAsync(scores_out, P->d_out, dg->m * 4, hipMemcpyDeviceToHost, s));
+  if (tr) {
+    const uint64_t n = dg->n, m = dg->m;
+    auto d2h = [&](void* dst, const void* src, uint64_t bytes) -> int {
+      if (dst) TGTP_HIP_CHECK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+      return TGTP_OK;
+    };
+    if ((rc = d2h(tr->taus, P->d_taus, m * 4)) || (rc = d2h(tr->nr, P->d_nr, m * 8)) ||
+        (rc = d2h(tr->walk_sum, A.tr_x, m * 4)) || (rc = d2h(tr->ell, A.tr_ell, n * 4)) ||
+        (rc = d2h(tr->edge_max, A.tr_emax, n * 4)) || (rc = d2h(tr->gmax, A.tr_gmax, n * 4)) ||
+        (rc = d2h(tr->gmin, A.tr_gmin, n * 4)) || (rc = d2h(tr->q_size, A.tr_qsize, n * 4)))
+      return rc;
+  }
+  TGTP_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  if (tr && tr->preds_row) {
+    std::vector<double> pd(dg->m);
+    TGTP_HIP_CHECK(ctx, hipMemcpy(pd.data(), P->d_preds, dg->m * 8, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < dg->m; ++i) tr->preds_row[i] = static_cast<float>(pd[i]);
+  }
+  if ((rc = check_status(ctx, P))) return rc;
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    float a = 0, b = 0, c = 0;
+    TGTP_HIP_CHECK(ctx, hipEventElapsedTime(&a, ctx->ev[4], ctx->ev[7]));
+    TGTP_HIP_CHECK(ctx, hipEventElapsedTime(&b, ctx->ev[4], ctx->ev[5]));
+    TGTP_HIP_CHECK(ctx, hipEventElapsedTime(&c, ctx->ev[5], ctx->ev[6]));
+    stats->kernel_ms = a;
+    stats->taus_ms = b;
+    stats->diffuse_ms = c;
+    stats->e2e_ms = tgtp_now_ms() - t0;
+  }
+  return TGTP_OK;
+}
+
+int tgtp_run_taus_fast(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e, float eps_half,
+                       int32_t mode, int32_t* d_taus) {
+  if (e->omega > 128 || e->omega < 3)
+    return set_error(ctx, TGTP_E_UNSUPPORTED, "fast modes need 3 <= omega <= 128");
+  tgtp_params p{static_cast<double>(eps_half) * 2.0, 1.0 / static_cast<double>(dg->n), 10, mode};
+  FastPlan* P = nullptr;
+  int rc = single_plan(ctx, dg, e, &p, &P);
+  if (rc) return rc;
+  // float(p.epsilon / 2) == eps_half exactly (doubling and halving are exact)
+  if ((rc = launch_fill_i32(ctx, P->d_taus, P->m, 0)) ||
+      (rc = launch_taus_fast(ctx, plan_tau_args(P, dg), mode == TGTP_MODE_FAST_BF16)))
+    return rc;
+  TGTP_HIP_CHECK(ctx, hipMemcpyAsync(d_taus, P->d_taus, dg->m * 4, hipMemcpyDeviceToDevice,
+                                     ctx->stream));
+  return TGTP_OK;
+}
+
+// ---- batched C ABI ---------------------------------------------------------------
+
+struct tgtp_batch {
+  int32_t count = 0;
+  int32_t mode = 0;
+  int32_t flags = 0;
+  tgtp_dgraph* dg = nullptr;          // the union (FAST modes)
+  FastPlan* plan = nullptr;
+  std::vector<uint64_t> eoff;         // CSR offset of each graph in the union
+  std::vector<uint64_t> m;            // CSR entries of each graph
+  // FAITHFUL mode: graphs run one after another
+  std::vector<tgtp_dgraph*> singles;
+  std::vector<tgtp_eigens> eig;
+  std::vector<tgtp_params> prm;
+  float tune_ms = 0;
+};
+
+extern "C" void tgtp_batch_destroy(tgtp_ctx* ctx, tgtp_batch* b) {
+  if (!b) return;
+  if (b->plan) free_plan(b->plan);
+  if (b->dg) tgtp_graph_free(ctx, b->dg);
+  for (auto* d : b->singles) tgtp_graph_free(ctx, d);
+  delete b;
+}
+
+extern "C" int tgtp_batch_create(tgtp_ctx* ctx, int32_t count, const tgtp_graph* graphs,
+                                 const tgtp_eigens* eigens, const tgtp_params* params,
+                                 int32_t flags, tgtp_batch** out) {
+  if (!ctx || !out) return TGTP_E_ARG;
+  *out = nullptr;
+  if (count < 1 || !graphs || !eigens || !params)
+    return set_error(ctx, TGTP_E_ARG, "empty batch or null arrays");
+  const int32_t mode = params[0].mode;
+  const int32_t omega = eigens[0].omega;
+  for (int32_t k = 0; k < count; ++k) {
+    const tgtp_params& p = params[k];
+    if (!(p.epsilon > 0 && p.epsilon < 1) || p.gamma < 1 || !(p.delta > 0))
+      return set_error(ctx, TGTP_E_ARG,
+                       "require 0 < epsilon < 1, integer gamma >= 1, delta > 0 for every graph");
+    if (p.mode != mode) return set_error(ctx, TGTP_E_ARG, "all graphs of a batch need one mode");
+    if (!graphs[k].idx || graphs[k].n == 0 || !eigens[k].values || !eigens[k].vectors ||
+        eigens[k].omega < 2 || static_cast<uint64_t>(eigens[k].omega) > graphs[k].n)
+      return set_error(ctx, TGTP_E_ARG, "graph or eigenpairs missing / omega out of range");
+    for (uint64_t v = 0; v < graphs[k].n; ++v)
+      if (graphs[k].idx[v + 1] == graphs[k].idx[v])
+        return set_error(ctx, TGTP_E_ARG,
+                         "graph vertex IDs must be contiguous with no isolated vertices");
+  }
+  TGTP_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  auto* b = new (std::nothrow) tgtp_batch();
+  if (!b) return set_error(ctx, TGTP_E_NOMEM, "host allocation");
+  b->count = count;
+  b->mode = mode;
+  b->flags = flags;
+  auto fail = [&](int rc) {
+    tgtp_batch_destroy(ctx, b);
+    return rc;
+  };
+  int rc;
+  if (mode == TGTP_MODE_FAITHFUL) {
+    for (int32_t k = 0; k < count; ++k) {
+      tgtp_dgraph* d = nullptr;
+      if ((rc = tgtp_graph_upload(ctx, &graphs[k], &d))) return fail(rc);
+      b->singles.push_back(d);
+      b->eig.push_back(eigens[k]);
+      b->prm.push_back(params[k]);
+      b->m.push_back(graphs[k].m);
+    }
+    *out = b;
+    return TGTP_OK;
+  }
+  for (int32_t k = 0; k < count; ++k)
+    if (eigens[k].omega != omega)
+      return fail(set_error(ctx, TGTP_E_UNSUPPORTED, "fast batches need one omega for all graphs"));
+  if (omega < 3 || omega > 128)
+    return fail(set_error(ctx, TGTP_E_UNSUPPORTED, "fast modes need 3 <= omega <= 128"));
+  // disjoint union on the host
+  uint64_t N = 0, M = 0;
+  for (int32_t k = 0; k < count; ++k) {
+    N += graphs[k].n;
+    M += graphs[k].m;
+  }
+  if (N > 0xFFFFFFFEull || M > 0xFFFFFFFFull)
+    return fail(set_error(ctx, TGTP_E_UNSUPPORTED, "batch exceeds 32-bit CSR capacity"));
+  std::vector<uint32_t> uidx(N + 1), uadj(M);
+  std::vector<float> ux(N * static_cast<uint64_t>(omega));
+  std::vector<GraphSpec> specs(count);
+  uint64_t vo = 0, eo = 0;
+  for (int32_t k = 0; k < count; ++k) {
+    const tgtp_graph& g = graphs[k];
+    for (uint64_t v = 0; v < g.n; ++v) uidx[vo + v] = static_cast<uint32_t>(eo + g.idx[v]);
+    for (uint64_t p = 0; p < g.m; ++p) uadj[eo + p] = static_cast<uint32_t>(vo + g.adj[p]);
+    std::memcpy(ux.data() + vo * omega, eigens[k].vectors, g.n * omega * sizeof(float));
+    specs[k] = GraphSpec{static_cast<uint32_t>(vo), static_cast<uint32_t>(g.n), eo, g.m,
+                         eigens[k].values, params[k].epsilon, params[k].delta, params[k].gamma};
+    b->eoff.push_back(eo);
+    b->m.push_back(g.m);
+    vo += g.n;
+    eo += g.m;
+  }
+  uidx[N] = static_cast<uint32_t>(M);
+  const tgtp_graph ug{N, M, uidx.data(), uadj.data()};
+  if ((rc = tgtp_graph_upload(ctx, &ug, &b->dg))) return fail(rc);
+  if ((rc = tgtp_ensure_prep(ctx, b->dg))) return fail(rc);
+  if ((rc = dev_alloc(ctx, &b->dg->X, ux.size(), "eigenvectors"))) return fail(rc);
+  TGTP_HIP_CHECK(ctx, hipMemcpyAsync(b->dg->X, ux.data(), ux.size() * 4, hipMemcpyHostToDevice,
+                                     ctx->stream));
+  b->dg->eig_omega = omega;
+  b->dg->eig_key = b;  // never matches a caller pointer
+  if ((rc = build_plan(ctx, b->dg, specs, omega, mode, &b->plan))) return fail(rc);
+  auto* prep = static_cast<TgtpPrep*>(b->dg->prep);
+  if (flags & TGTP_BATCH_AUTOTUNE) {
+    const double t0 = tgtp_now_ms();
+    if ((rc = autotune(ctx, b->plan, b->dg, prep->d_src_of))) return fail(rc);
+    b->tune_ms = static_cast<float>(tgtp_now_ms() - t0);
+  }
+  if (flags & TGTP_BATCH_GRAPH)
+    if ((rc = capture(ctx, b->plan, b->dg, prep->d_src_of))) return fail(rc);
+  *out = b;
+  return TGTP_OK;
+}
+
+extern "C" int tgtp_batch_run(tgtp_ctx* ctx, tgtp_batch* b, float* const* scores_out,
+                              tgtp_stats* stats) {
+  if (!ctx) return TGTP_E_ARG;
+  if (!b || !scores_out) return set_error(ctx, TGTP_E_ARG, "null batch or outputs");
+  const double t0 = tgtp_now_ms();
+  TGTP_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  int rc;
+  if (b->mode == TGTP_MODE_FAITHFUL) {
+    double kms = 0;
+    for (int32_t k = 0; k < b->count; ++k) {
+      tgtp_stats st{};
+      if ((rc = tgtp_score_dev(ctx, b->singles[k], &b->eig[k], &b->prm[k], scores_out[k], &st)))
+        return rc;
+      kms += st.kernel_ms;
+    }
+    if (stats) {
+      std::memset(stats, 0, sizeof(*stats));
+      stats->kernel_ms = kms;
+      stats->e2e_ms = tgtp_now_ms() - t0;
+    }
+    return TGTP_OK;
+  }
+  hipStream_t s = ctx->stream;
+  auto* prep = static_cast<TgtpPrep*>(b->dg->prep);
+  TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[4], s));
+  if (b->plan->exec) {
+    TGTP_HIP_CHECK(ctx, hipGraphLaunch(b->plan->exec, s));
+  } else if ((rc = enqueue_run(ctx, b->plan, b->dg, prep->d_src_of, false, nullptr))) {
+    return rc;
+  }
+  TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[7], s));
+  for (int32_t k = 0; k < b->count; ++k)
+    if (scores_out[k] && b->m[k])
+      TGTP_HIP_CHECK(ctx, hipMemcpyAsync(scores_out[k], b->plan->d_out + b->eoff[k], b->m[k] * 4,
+                                         hipMemcpyDeviceToHost, s));
+  TGTP_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  if ((rc = check_status(ctx, b->plan))) return rc;
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    float a = 0;
+    TGTP_HIP_CHECK(ctx, hipEventElapsedTime(&a, ctx->ev[4], ctx->ev[7]));
+    stats->kernel_ms = a;
+    stats->e2e_ms = tgtp_now_ms() - t0;
+    stats->work_items = b->plan->n_work;
+    stats->taus_ms = b->tune_ms;
+  }
+  return TGTP_OK;
+}
+
+extern "C" int tgtp_batch_config(const tgtp_batch* b, uint32_t* light, uint32_t* small_bq,
+                                 uint32_t* slots, uint32_t* max_slots) {
+  if (!b || !b->plan) return TGTP_E_ARG;
+  if (light) *light = b->plan->light;
+  if (small_bq) *small_bq = b->plan->small_bq;
+  if (slots) *slots = b->plan->slots;
+  if (max_slots) *max_slots = b->plan->max_slots;
+  return TGTP_OK;
+}
+
+extern "C" int tgtp_score_batch(tgtp_ctx* ctx, int32_t count, const tgtp_graph* graphs,
+                                const tgtp_eigens* eigens, const tgtp_params* params,
+                                float* const* scores_out, tgtp_stats* stats) {
+  tgtp_batch* b = nullptr;
+  int rc = tgtp_batch_create(ctx, count, graphs, eigens, params, 0, &b);
+  if (rc) return rc;
+  rc = tgtp_batch_run(ctx, b, scores_out, stats);
+  tgtp_batch_destroy(ctx, b);
+  return rc;
+}

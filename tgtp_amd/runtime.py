@@ -41,6 +41,62 @@ class DeviceGraph:
             pass
 
 
+class Batch:
+    """A batch of graphs scored together (tgtp_batch_*).  Keeps the host
+    arrays alive for the batch's lifetime."""
+
+    def __init__(self, ctx: "Context", graphs, eigens, epsilon: float, gamma: int = 10,
+                 delta=None, mode: int = L.MODE_FAST_FP32, autotune: bool = False,
+                 graph: bool = False):
+        self.ctx = ctx
+        self.graphs = list(graphs)
+        k = len(self.graphs)
+        self._keep = []
+        gs = (L.GraphStruct * k)()
+        es = (L.EigensStruct * k)()
+        ps = (L.ParamsStruct * k)()
+        for i, (g, (lam, X)) in enumerate(zip(self.graphs, eigens)):
+            gs[i] = g.struct()
+            e, keep = Context._eigens(lam, X)
+            es[i] = e
+            self._keep.append((g, keep))
+            ps[i] = L.ParamsStruct(epsilon, (1.0 / g.n) if delta is None else delta, gamma, mode)
+        self._structs = (gs, es, ps)
+        h = C.c_void_p()
+        flags = (L.BATCH_AUTOTUNE if autotune else 0) | (L.BATCH_GRAPH if graph else 0)
+        _check(ctx.ptr, L.lib().tgtp_batch_create(ctx.ptr, k, gs, es, ps, flags, C.byref(h)),
+               "tgtp_batch_create")
+        self.ptr = h
+
+    def run(self):
+        """-> (list of float32 score arrays, stats)."""
+        outs = [np.zeros(g.m, dtype=np.float32) for g in self.graphs]
+        arr = (C.POINTER(C.c_float) * len(outs))(*[L.ptr(o, C.c_float) for o in outs])
+        st = L.StatsStruct()
+        _check(self.ctx.ptr, L.lib().tgtp_batch_run(self.ctx.ptr, self.ptr, arr, C.byref(st)),
+               "tgtp_batch_run")
+        return outs, st.as_dict()
+
+    def config(self) -> dict:
+        v = [C.c_uint32() for _ in range(4)]
+        rc = L.lib().tgtp_batch_config(self.ptr, *[C.byref(x) for x in v])
+        if rc != L.TGTP_OK:
+            return {}
+        return dict(zip(("light_degree", "small_bouquets", "slots", "max_slots"),
+                        [x.value for x in v]))
+
+    def close(self) -> None:
+        if self.ptr:
+            L.lib().tgtp_batch_destroy(self.ctx.ptr, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One HIP stream + scratch on one device (tgtp_ctx)."""
 
@@ -122,6 +178,33 @@ class Context:
         _check(self.ptr, rc, "tgtp_score")
         del keep
         return out, st.as_dict()
+
+    def batch(self, graphs, eigens, epsilon: float, gamma: int = 10, delta=None,
+              mode: int = L.MODE_FAST_FP32, autotune: bool = False, graph: bool = False) -> Batch:
+        return Batch(self, graphs, eigens, epsilon, gamma, delta, mode, autotune, graph)
+
+    def score_batch(self, graphs, eigens, epsilon: float, gamma: int = 10, delta=None,
+                    mode: int = L.MODE_FAST_FP32):
+        """One-shot tgtp_score_batch -> (list of score arrays, stats)."""
+        graphs = list(graphs)
+        k = len(graphs)
+        gs = (L.GraphStruct * k)()
+        es = (L.EigensStruct * k)()
+        ps = (L.ParamsStruct * k)()
+        keep = []
+        for i, (g, (lam, X)) in enumerate(zip(graphs, eigens)):
+            gs[i] = g.struct()
+            e, kp = self._eigens(lam, X)
+            es[i] = e
+            keep.append(kp)
+            ps[i] = L.ParamsStruct(epsilon, (1.0 / g.n) if delta is None else delta, gamma, mode)
+        outs = [np.zeros(g.m, dtype=np.float32) for g in graphs]
+        arr = (C.POINTER(C.c_float) * k)(*[L.ptr(o, C.c_float) for o in outs])
+        st = L.StatsStruct()
+        _check(self.ptr, L.lib().tgtp_score_batch(self.ptr, k, gs, es, ps, arr, C.byref(st)),
+               "tgtp_score_batch")
+        del keep
+        return outs, st.as_dict()
 
     def score_trace(self, dg: DeviceGraph, lam, X, epsilon: float, gamma: int = 10,
                     delta: float | None = None, mode: int = L.MODE_FAITHFUL):
