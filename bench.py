@@ -99,28 +99,27 @@ def bench_saba(args):
 def bench_tgtp(args):
     """tgtp() on the n=4096 Chung-Lu graph (omega=128, gamma=10): edges/s =
     CSR entries / seconds of one call with graph and eigenpairs resident in
-    HBM (benchmark.cpp:40-42 times tgtp() only).  Returns a dict of legs."""
+    HBM (benchmark.cpp:40-42 times tgtp() only).  Errors are against the
+    FP64 exact scores (tgtp_amd.exact_sc, dense grounded-Laplacian
+    Cholesky).  Returns a dict of legs."""
     import numpy as np
 
-    import oracle
-    from oracle.exact import exact_resistance
     from tgtp_amd import (MODE_FAITHFUL, MODE_FAST_BF16, MODE_FAST_FP32, Context,
-                          eigenprep as E, workloads as WL)
+                          eigenprep as E, exact_sc, workloads as WL)
 
     g = WL.chung_lu_4096()
     t0 = time.time()
     lam, X = E.eigens_f32(*E.compute_eigens(g, 128))
-    R, _ = exact_resistance(g.idx, g.adj)
-    log(f"[bench] tgtp graph n={g.n} m={g.m}, eigenpairs + exact oracle in {time.time() - t0:.1f}s")
+    R = exact_sc.exact_scores(g, "cuda:0")
+    log(f"[bench] tgtp graph n={g.n} m={g.m}, eigenpairs + exact scores in {time.time() - t0:.1f}s")
     out = {"graph": WL.describe(WL.CL4096), "n": g.n, "m": g.m, "omega": 128, "gamma": 10}
-    threads = host_threads()
     modes = ((MODE_FAITHFUL, "faithful"), (MODE_FAST_FP32, "fast_fp32"),
              (MODE_FAST_BF16, "fast_bf16"))
     with Context(0) as ctx:
         dg = ctx.upload(g)
         for eps in (0.01, 0.05):
             leg = {}
-            ref = oracle.tgtp(g.idx, g.adj, lam, X, eps, 10, threads=threads)
+            faithful_scores = None
             for mode, name in modes:
                 s, st = ctx.score(dg, lam, X, eps, 10, mode=mode)  # warmup + eigen upload
                 times, kms = [], []
@@ -129,36 +128,161 @@ def bench_tgtp(args):
                     s, st = ctx.score(dg, lam, X, eps, 10, mode=mode)
                     times.append(time.perf_counter() - t1)
                     kms.append(st["kernel_ms"])
+                if mode == MODE_FAITHFUL:
+                    faithful_scores = s
                 best = min(times)
-                err = np.abs(s.astype(np.float64) - R)
                 leg[name] = {"edges_per_s": g.m / best, "seconds": best,
                              "kernel_ms": min(kms), "taus_ms": st["taus_ms"],
                              "diffuse_ms": st["diffuse_ms"], "host_fixups": st["host_fixups"],
-                             "bitwise_equal_cpu": bool(np.array_equal(s.view(np.uint32),
-                                                                      ref.view(np.uint32))),
-                             "vs_exact_fp64": {"max_abs": float(err.max()),
-                                               "mean_abs": float(err.mean()),
-                                               "max_rel": float((err / R).max()),
-                                               "mean_rel": float((err / R).mean()),
-                                               "within_eps": bool(err.max() <= eps)}}
+                             "vs_exact_fp64": error_stats(s, R, eps)}
             # PCIe-inclusive: host arrays in, scores out (tgtp_score)
             t1 = time.perf_counter()
             ctx.score(g, lam, X, eps, 10)
             leg["faithful"]["e2e_with_h2d_edges_per_s"] = g.m / (time.perf_counter() - t1)
-            ctimes = []
-            for _ in range(3):
-                t1 = time.perf_counter()
-                oracle.tgtp(g.idx, g.adj, lam, X, eps, 10, threads=threads)
-                ctimes.append(time.perf_counter() - t1)
-            leg["cpu_baseline"] = {"edges_per_s": g.m / min(ctimes), "seconds": min(ctimes),
-                                   "cores": threads, "kind": "port",
-                                   "sample": "full run, min of 3 (oracle C++ OpenMP, "
-                                             "schedule(dynamic) over sources)"}
+            if not args.no_cpu:
+                leg["cpu_baseline"] = cpu_tgtp_baseline([g], [(lam, X)], eps, [faithful_scores])
             out[f"eps_{eps}"] = leg
             log(f"[bench] tgtp eps={eps}: " + ", ".join(
                 f"{nm} {leg[nm]['edges_per_s']:.3e} edges/s (max err {leg[nm]['vs_exact_fp64']['max_abs']:.4f})"
-                for _, nm in modes) + f"; cpu {leg['cpu_baseline']['edges_per_s']:.3e}")
+                for _, nm in modes) + (f"; cpu {leg['cpu_baseline']['edges_per_s']:.3e}"
+                                       if "cpu_baseline" in leg else ""))
         dg.free()
+    return out
+
+
+def error_stats(scores, exact, eps, pos=None):
+    import numpy as np
+
+    s = scores.astype(np.float64) if pos is None else scores[pos].astype(np.float64)
+    err = np.abs(s - exact)
+    rel = err / exact
+    return {"max_abs": float(err.max()), "mean_abs": float(err.mean()),
+            "max_rel": float(rel.max()), "mean_rel": float(rel.mean()),
+            "within_eps": bool(err.max() <= eps), "edges_checked": int(err.shape[0])}
+
+
+def cpu_tgtp_baseline(graphs, eigens, eps, gpu_faithful=None):
+    """The oracle's tgtp (C++ OpenMP over sources, the reference's
+    algorithm) on the host cores, min of 3 per graph; when the GPU faithful
+    scores are given, whether they equal the oracle's bit for bit."""
+    import numpy as np
+
+    import oracle
+
+    threads = host_threads()
+    total_m, total_s, equal = 0, 0.0, True
+    for k, (g, (lam, X)) in enumerate(zip(graphs, eigens)):
+        best = None
+        for _ in range(3):
+            t1 = time.perf_counter()
+            ref = oracle.tgtp(g.idx, g.adj, lam, X, eps, 10, threads=threads)
+            dt = time.perf_counter() - t1
+            best = dt if best is None else min(best, dt)
+        if gpu_faithful is not None and gpu_faithful[k] is not None:
+            equal &= bool(np.array_equal(ref.view(np.uint32), gpu_faithful[k].view(np.uint32)))
+        total_m += g.m
+        total_s += best
+    out = {"edges_per_s": total_m / total_s, "seconds": total_s, "cores": threads,
+           "kind": "port",
+           "sample": f"{len(graphs)} graph(s), {total_m} CSR entries, min of 3 per graph "
+                     "(oracle C++ OpenMP, schedule(dynamic) over sources)"}
+    if gpu_faithful is not None:
+        out["gpu_faithful_bitwise_equal"] = equal
+    return out
+
+
+def bench_batch(args):
+    """Batched tgtp on 256 seeded Chung-Lu graphs (n log-uniform in
+    [1024, 65536], exponent U(2.1, 3.0), mean degree U(8, 40)), omega=128,
+    gamma=10: one autotuned, graph-captured batch run vs one plain
+    tgtp_score_dev call per graph, both with graphs and eigenpairs resident
+    in HBM and scores copied back to the host.  edges/s = total CSR entries /
+    seconds.  Errors are against FP64 exact resistances of sampled edges
+    (block PCG, tgtp_amd.exact_sc)."""
+    import numpy as np
+    import torch
+
+    from tgtp_amd import MODE_FAST_BF16, MODE_FAST_FP32, Context, eigenprep as E, exact_sc
+    from tgtp_amd import workloads as WL
+
+    t0 = time.time()
+    graphs = WL.batch_graphs(args.batch_count)
+    total_m = sum(g.m for g in graphs)
+    log(f"[bench] batch: {len(graphs)} graphs, sum n={sum(g.n for g in graphs)} "
+        f"sum m={total_m}, generated in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    eig = []
+    for k, g in enumerate(graphs):
+        eig.append(E.eigens_f32(*E.compute_eigens_device(g, 128, device="cuda:0")))
+        if k % 32 == 31:
+            log(f"[bench] batch: eigenpairs {k + 1}/{len(graphs)} ({time.time() - t0:.1f}s)")
+    t_eig = time.time() - t0
+    log(f"[bench] batch: eigenpairs (block Krylov, FP64 on the GPU) in {t_eig:.1f}s")
+    t0 = time.time()
+    rng = np.random.default_rng(7)
+    samples = [np.sort(rng.choice(g.m, size=min(args.batch_sample_edges, g.m), replace=False))
+               for g in graphs]
+    exact = [exact_sc.sampled_scores(g, p, "cuda:0") for g, p in zip(graphs, samples)]
+    torch.cuda.empty_cache()
+    log(f"[bench] batch: exact resistances of {sum(len(p) for p in samples)} sampled edges "
+        f"in {time.time() - t0:.1f}s")
+    out = {"workload": "256 Chung-Lu graphs " + WL.describe(
+               {k: v for k, v in WL.BATCH256.items() if k != "count"}),
+           "graphs": len(graphs), "sum_n": sum(g.n for g in graphs), "sum_m": total_m,
+           "omega": 128, "gamma": 10, "eigen_prep_s": round(t_eig, 1)}
+    reps = max(3, args.steps)
+    with Context(0) as ctx:
+        for eps in (0.05, 0.01):
+            leg = {}
+            for mode, name in ((MODE_FAST_FP32, "fast_fp32"), (MODE_FAST_BF16, "fast_bf16")):
+                plain_s, plain_out = 0.0, []
+                for g, e in zip(graphs, eig):
+                    dg = ctx.upload(g)
+                    ctx.score(dg, *e, eps, 10, mode=mode)  # eigen upload + plan
+                    best = None
+                    for _ in range(reps):
+                        t1 = time.perf_counter()
+                        s, _ = ctx.score(dg, *e, eps, 10, mode=mode)
+                        dt = time.perf_counter() - t1
+                        best = dt if best is None else min(best, dt)
+                    plain_s += best
+                    plain_out.append(s)
+                    dg.free()
+                t1 = time.perf_counter()
+                b = ctx.batch(graphs, eig, eps, 10, mode=mode, autotune=True, graph=True)
+                t_create = time.perf_counter() - t1
+                outs, st = b.run()
+                times, kms = [], []
+                for _ in range(reps):
+                    t1 = time.perf_counter()
+                    outs, st = b.run()
+                    times.append(time.perf_counter() - t1)
+                    kms.append(st["kernel_ms"])
+                cfg = b.config()
+                b.close()
+                best = min(times)
+                bit_equal = all(np.array_equal(a.view(np.uint32), p.view(np.uint32))
+                                for a, p in zip(outs, plain_out))
+                est = np.concatenate([o[p] for o, p in zip(outs, samples)])
+                leg[name] = {"batched_edges_per_s": total_m / best, "batched_seconds": best,
+                             "batched_kernel_ms": min(kms),
+                             "plain_edges_per_s": total_m / plain_s, "plain_seconds": plain_s,
+                             "batched_over_plain": plain_s / best,
+                             "create_autotune_capture_s": round(t_create, 2),
+                             "autotuned": cfg, "bitwise_equal_plain": bit_equal,
+                             "vs_exact_fp64_sampled": error_stats(est, np.concatenate(exact),
+                                                                  eps)}
+                log(f"[bench] batch eps={eps} {name}: batched {total_m / best:.3e} edges/s "
+                    f"({best * 1e3:.1f} ms), plain {total_m / plain_s:.3e} "
+                    f"({plain_s * 1e3:.1f} ms), x{plain_s / best:.2f}, cfg {cfg}, "
+                    f"max err {leg[name]['vs_exact_fp64_sampled']['max_abs']:.4f}")
+            if not args.no_cpu:
+                sub = list(range(0, len(graphs), max(1, len(graphs) // args.batch_cpu_graphs)))
+                leg["cpu_baseline"] = cpu_tgtp_baseline([graphs[i] for i in sub],
+                                                        [eig[i] for i in sub], eps)
+                leg["cpu_baseline"]["sample"] += f"; every {max(1, len(graphs) // args.batch_cpu_graphs)}th graph of the batch"
+                log(f"[bench] batch eps={eps} cpu {leg['cpu_baseline']['edges_per_s']:.3e} edges/s")
+            out[f"eps_{eps}"] = leg
     return out
 
 
@@ -196,6 +320,10 @@ def main() -> int:
     ap.add_argument("--cpu-sample-sources", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tgtp", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--batch-sample-edges", type=int, default=32)
+    ap.add_argument("--batch-cpu-graphs", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
@@ -216,6 +344,8 @@ def main() -> int:
     del g
     if not args.no_tgtp:
         line["tgtp"] = bench_tgtp(args)
+    if not args.no_batch:
+        line["tgtp_batch"] = bench_batch(args)
     print(json.dumps(line), flush=True)
     return 0
 

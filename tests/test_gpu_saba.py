@@ -96,6 +96,21 @@ def test_layouts_and_lds_staging_agree(gpu_ctx, monkeypatch, packed, hot):
     _check(gpu_ctx, W.chung_lu_4096(), 9, 512, 5, 4000, label=f"packed={packed} hot={hot}")
 
 
+@pytest.mark.parametrize("knobs", [
+    {"TGTP_WALK_BUCKETS": "0"},                             # direct atomics
+    {"TGTP_WALK_REC_MIB": "1"},                             # many source chunks
+    {"TGTP_WALK_REC_PCT": "5"},                             # regions overflow: atomics
+    {"TGTP_WALK_HOT": "1000", "TGTP_WALK_PACKED": "0"},     # 2 buckets, plain layout
+])
+def test_bucketed_counting_paths(gpu_ctx, monkeypatch, knobs):
+    """Cold visits staged per vertex-range bucket and counted by the LDS
+    histogram pass give the same counters as direct atomics, through source
+    chunking and region overflow."""
+    for k, v in knobs.items():
+        monkeypatch.setenv(k, v)
+    _check(gpu_ctx, G.chung_lu(70000, 2.3, 12.0, seed=9), 7, 64, label=f"buckets {knobs}")
+
+
 def test_deterministic_repeat(gpu_ctx):
     g = W.chung_lu_4096()
     dg = gpu_ctx.upload(g)

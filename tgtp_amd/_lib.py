@@ -85,6 +85,18 @@ def lib() -> C.CDLL:
         return _lib
     if not LIB_PATH.exists():
         raise ImportError(f"{LIB_PATH} is not built; run `python -m tgtp_amd.build`")
+    # One HIP runtime per process.  torch ships its own libamdhip64 /
+    # libhsa-runtime64 (SONAMEs libamdhip64.so.7 / libhsa-runtime64.so.1,
+    # but torch asks for them by unversioned names), so if this library
+    # loaded /opt/rocm's copies first, a later torch import would bring a
+    # second HSA runtime into the process and torch would find no GPU.
+    # Importing torch first makes this library bind to the runtime torch
+    # already loaded (the dynamic loader matches our versioned NEEDED
+    # entries against those SONAMEs).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(str(LIB_PATH))
     P = C.POINTER
     vp = C.c_void_p

@@ -89,6 +89,8 @@ enum BufSlot {
   kBufMisc = 9,
   kBufFlags = 10,
   kBufTrace = 11,
+  kBufWalkRec = 12,
+  kBufWalkFill = 13,
 };
 
 // Returns a device buffer of at least `bytes` in slot `slot` (grow-only).

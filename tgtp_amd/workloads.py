@@ -33,3 +33,28 @@ def grid_512():
 def describe(params: dict) -> str:
     return ",".join(f"{k}={v:.6g}" if isinstance(v, float) else f"{k}={v}"
                     for k, v in params.items())
+
+
+# Batched tgtp workload: 256 Chung-Lu graphs, n = floor(2^U(10, 16))
+# (log-uniform in [1024, 65536]), exponent U(2.1, 3.0), mean degree U(8, 40),
+# graph k seeded 1000 + k; the parameters come from one seeded PCG64 stream.
+BATCH256 = dict(count=256, log2n=(10.0, 16.0), beta=(2.1, 3.0), mean_deg=(8.0, 40.0),
+                seed=5, graph_seed0=1000)
+
+
+def batch_params(count: int | None = None, spec: dict = BATCH256):
+    """[(n, beta, mean_deg, seed)] of the batched workload (first `count`)."""
+    import numpy as np
+
+    rng = np.random.default_rng(spec["seed"])
+    k = spec["count"]
+    e = rng.uniform(*spec["log2n"], size=k)
+    b = rng.uniform(*spec["beta"], size=k)
+    d = rng.uniform(*spec["mean_deg"], size=k)
+    out = [(int(np.floor(2.0 ** e[i])), float(b[i]), float(d[i]), spec["graph_seed0"] + i)
+           for i in range(k)]
+    return out if count is None else out[:count]
+
+
+def batch_graphs(count: int | None = None, spec: dict = BATCH256):
+    return [G.chung_lu(n, beta, md, seed=s) for n, beta, md, s in batch_params(count, spec)]
