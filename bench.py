@@ -211,13 +211,12 @@ def bench_batch(args):
     log(f"[bench] batch: {len(graphs)} graphs, sum n={sum(g.n for g in graphs)} "
         f"sum m={total_m}, generated in {time.time() - t0:.1f}s")
     t0 = time.time()
-    eig = []
-    for k, g in enumerate(graphs):
-        eig.append(E.eigens_f32(*E.compute_eigens_device(g, 128, device="cuda:0")))
-        if k % 32 == 31:
-            log(f"[bench] batch: eigenpairs {k + 1}/{len(graphs)} ({time.time() - t0:.1f}s)")
+    eig = [E.eigens_f32(*e) for e in E.compute_eigens_many(
+        graphs, 128, device="cuda:0",
+        progress=lambda done, tot: log(f"[bench] batch: eigenpairs {done}/{tot} "
+                                       f"({time.time() - t0:.1f}s)"))]
     t_eig = time.time() - t0
-    log(f"[bench] batch: eigenpairs (block Krylov, FP64 on the GPU) in {t_eig:.1f}s")
+    log(f"[bench] batch: eigenpairs (batched block Krylov, FP64 on the GPU) in {t_eig:.1f}s")
     t0 = time.time()
     rng = np.random.default_rng(7)
     samples = [np.sort(rng.choice(g.m, size=min(args.batch_sample_edges, g.m), replace=False))
@@ -230,13 +229,15 @@ def bench_batch(args):
                {k: v for k, v in WL.BATCH256.items() if k != "count"}),
            "graphs": len(graphs), "sum_n": sum(g.n for g in graphs), "sum_m": total_m,
            "omega": 128, "gamma": 10, "eigen_prep_s": round(t_eig, 1)}
-    reps = max(3, args.steps)
+    reps = 2
     with Context(0) as ctx:
         for eps in (0.05, 0.01):
             leg = {}
             for mode, name in ((MODE_FAST_FP32, "fast_fp32"), (MODE_FAST_BF16, "fast_bf16")):
                 plain_s, plain_out = 0.0, []
-                for g, e in zip(graphs, eig):
+                for gi, (g, e) in enumerate(zip(graphs, eig)):
+                    if gi % 64 == 63:
+                        log(f"[bench] batch eps={eps} {name}: plain {gi + 1}/{len(graphs)}")
                     dg = ctx.upload(g)
                     ctx.score(dg, *e, eps, 10, mode=mode)  # eigen upload + plan
                     best = None

@@ -42,3 +42,14 @@ def test_device_eigens_large_sparse():
     w, X = E.compute_eigens(g, 128)           # ARPACK
     w1, X1 = E.compute_eigens_device(g, 128, device="cuda:0")
     assert np.abs(w1 - w).max() < 1e-9
+
+
+@pytest.mark.parametrize("dev", ["cpu", pytest.param("cuda:0", marks=pytest.mark.gpu)])
+def test_batched_size_class_matches_per_graph(dev):
+    """Graphs of different sizes solved as one padded block-diagonal class
+    give each graph its own eigenpairs."""
+    gs = [G.chung_lu(n, b, d, seed=s) for n, b, d, s in
+          [(1100, 2.5, 10.0, 1), (1300, 2.2, 30.0, 2), (1500, 2.9, 8.0, 3), (2400, 2.5, 20.0, 4)]]
+    res = E.compute_eigens_many(gs, 48, device=dev)
+    for g, (w1, X1) in zip(gs, res):
+        _check(w1, X1, *E.compute_eigens(g, 48))

@@ -119,3 +119,17 @@ def test_fast_on_rewired_grid_tracks_faithful(gpu_ctx):
              f"mean {err.mean():.5f} (faithful {ferr.mean():.5f})")
         assert err.max() <= ferr.max() + 0.05
         assert err.mean() <= ferr.mean() + 0.025
+
+
+def test_fast_subsets_do_not_change_scores(gpu_ctx, monkeypatch):
+    """Sources advance in blocks of 64 whose states share subset buffers; the
+    scores do not depend on how blocks are grouped into subsets."""
+    g = G.chung_lu(3000, 2.4, 14.0, seed=21)
+    lam, X = E.eigens_f32(*E.compute_eigens(g, 32))
+    for mode, name in MODES:
+        a, _ = gpu_ctx.score(g, lam, X, 0.02, 10, mode=mode)
+        monkeypatch.setenv("TGTP_FAST_STATE_MIB", "1")  # one block per subset
+        b, _ = gpu_ctx.score(g, lam, X, 0.02, 10, mode=mode)
+        monkeypatch.delenv("TGTP_FAST_STATE_MIB")
+        _say(f"  subsets {name}: equal={np.array_equal(a.view(np.uint32), b.view(np.uint32))}")
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))

@@ -1,0 +1,543 @@
+// fast_block.hip -- blocked FAST_FP32 / FAST_BF16 per-source pipeline.
+//
+// Same algorithm as AESC::tgtp (aesc.h:444-551) with the round structure
+// made synchronous:
+//   * diffusion round l -> l+1 (the synchronous form of aesc.h:219-239):
+//     pvec'[u] = (sum_{w in adj(u)} pvec[w]) / deg(u), so pvec_l = P^l e_s
+//     (P = D^-1 A), the series TGT+ truncates.  The frontier of round l is
+//     the support of pvec_l.  64 sources advance together: their vectors are
+//     the columns of an n x 64 block, a round is one sparse x dense product,
+//     and each neighbour read is one 256-byte row (one float per source),
+//     instead of one random 4-byte read per source.  Each sum runs in
+//     adjacency order in one lane: deterministic, and independent of which
+//     sources share a block.
+//   * stopping rule per source as the reference (RWRATIO 25, aesc.h:493-503),
+//     with the round's max / min / size / degree sum of the frontier; a
+//     stopped source's column is carried unchanged through later rounds.
+//   * top-gamma bound (ties at the gamma-th value keep every tied vertex),
+//     chi and walk counts (aesc.h:392-442, 505-521), then the 2-way SABA
+//     walks with the reference's grid / hash / fallback per bouquet; each
+//     bouquet's 8 lane sums are added in FP64, bouquets in a fixed order;
+//     per-edge row sums (preds) are FP64 throughout.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+
+#include "common.hpp"
+#include "fast_block.hpp"
+#include "wave.hpp"
+
+using namespace tgtp_dev;
+
+namespace {
+
+constexpr int kPB = 256;               // post kernel: threads per source
+constexpr int kPW = kPB / 64;
+constexpr uint32_t kPostLds = 12288;   // floats of a source vector staged in LDS
+constexpr uint32_t kTopK = 16;         // single-pass top-gamma for gamma <= 16
+constexpr double kMaxWalksFast = 68719476736.0;
+constexpr uint32_t kInfBits = 0x7f800000u;
+
+// Last block in [b0, b1) whose first round tile is <= t.
+__device__ __forceinline__ uint32_t find_block(const BlockInfo* B, uint32_t b0, uint32_t b1,
+                                               uint32_t t) {
+  uint32_t lo = b0, hi = b1;
+  while (hi - lo > 1) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (B[mid].tile0 <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__global__ void blk_zero_kernel(float* __restrict__ x, uint64_t n) {
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += gridDim.x * 256ull) x[i] = 0.0f;
+}
+
+// One wave per source column: column state, e_s, preds row = 1/d_s.
+__global__ void __launch_bounds__(256) blk_init_kernel(BlockArgs A, uint32_t b0, uint32_t b1) {
+  const uint32_t c = blockIdx.x * 4u + wave_id();
+  const uint32_t b = b0 + c / kBlkCols, j = c % kBlkCols;
+  if (b >= b1) return;
+  const BlockInfo B = A.blocks[b];
+  const uint32_t col = b * kBlkCols + j;
+  if (lane_id() == 0) {
+    A.st_mx[col] = 0u;
+    A.st_mn[col] = kInfBits;
+    A.st_q[col] = 0u;
+    A.st_dsum[col] = 0ull;
+    A.ell[col] = 0;
+    A.gmax[col] = 0.0f;
+    A.gmin[col] = 0.0f;
+    A.qsize[col] = 0u;
+    A.active[col] = j < B.cnt ? 1 : 0;
+    if (j == 0) A.last[b] = 0u;
+  }
+  if (j >= B.cnt) return;
+  const uint32_t s = B.src0 + j;
+  if (lane_id() == 0) A.xa[B.xoff + static_cast<uint64_t>(s) * kBlkCols + j] = 1.0f;
+  const uint2 r = A.rowd[B.voff + s];
+  const double dsd = static_cast<double>(r.y);
+  for (uint32_t p = r.x + lane_id(); p < r.x + r.y; p += 64) A.preds_d[p] = 1.0 / dsd;
+}
+
+// One round for every tile (64 vertices of one block) of the subset: lane j
+// computes source j's new value at each vertex of the tile.
+__global__ void __launch_bounds__(256)
+blk_round_kernel(BlockArgs A, uint32_t b0, uint32_t b1, const float* __restrict__ cur,
+                 float* __restrict__ nxt) {
+  __shared__ float s_mx[4][kBlkCols];
+  __shared__ float s_mn[4][kBlkCols];
+  __shared__ uint32_t s_q[4][kBlkCols];
+  __shared__ unsigned long long s_ds[4][kBlkCols];
+  const uint32_t t = blockIdx.x;
+  const uint32_t b = find_block(A.blocks, b0, b1, t);
+  const BlockInfo B = A.blocks[b];
+  const uint32_t j = lane_id(), wv = wave_id();
+  const uint32_t col = b * kBlkCols + j;
+  const bool act = A.active[col] != 0;
+  float mx = 0.0f, mn = INFINITY;
+  uint32_t q = 0;
+  unsigned long long ds = 0;
+  if (__ballot(act) != 0ull) {  // a block whose sources all stopped is left alone
+    const uint32_t row0 = (t - B.tile0) * kBlkRows;
+    const uint32_t rend = min(row0 + kBlkRows, B.n);
+    const float* Xc = cur + B.xoff;
+    float* Xn = nxt + B.xoff;
+    for (uint32_t u = row0 + wv; u < rend; u += 4) {
+      const uint64_t ro = static_cast<uint64_t>(u) * kBlkCols + j;
+      if (act) {
+        const uint2 r = A.rowd[B.voff + u];
+        float sum = 0.0f;
+#pragma unroll 8
+        for (uint32_t k = 0; k < r.y; ++k)
+          sum += Xc[static_cast<uint64_t>(A.adj[r.x + k] - B.voff) * kBlkCols + j];
+        const float val = sum / A.degf[B.voff + u];
+        Xn[ro] = val;
+        if (val > 0.0f) {
+          mx = fmaxf(mx, val);
+          mn = fminf(mn, val);
+          ++q;
+          ds += r.y;
+        }
+      } else {
+        Xn[ro] = Xc[ro];
+      }
+    }
+  }
+  s_mx[wv][j] = mx;
+  s_mn[wv][j] = mn;
+  s_q[wv][j] = q;
+  s_ds[wv][j] = ds;
+  __syncthreads();
+  if (wv == 0) {
+#pragma unroll
+    for (int w = 1; w < 4; ++w) {
+      mx = fmaxf(mx, s_mx[w][j]);
+      mn = fminf(mn, s_mn[w][j]);
+      q += s_q[w][j];
+      ds += s_ds[w][j];
+    }
+    if (act && q) {
+      atomicMax(&A.st_mx[col], __float_as_uint(mx));  // non-negative floats order as bits
+      atomicMin(&A.st_mn[col], __float_as_uint(mn));
+      atomicAdd(&A.st_q[col], q);
+      atomicAdd(&A.st_dsum[col], ds);
+    }
+  }
+}
+
+// After a round: per active source, the round's row update of preds and the
+// reference's cost rule (aesc.h:493-503).  One wave per source column.
+__global__ void __launch_bounds__(256)
+blk_stop_kernel(BlockArgs A, uint32_t b0, uint32_t b1, const float* __restrict__ xn) {
+  const uint32_t c = blockIdx.x * 4u + wave_id();
+  const uint32_t b = b0 + c / kBlkCols, j = c % kBlkCols;
+  if (b >= b1) return;
+  const BlockInfo B = A.blocks[b];
+  const uint32_t col = b * kBlkCols + j;
+  if (j >= B.cnt || A.active[col] == 0) return;
+  const float mx = __uint_as_float(A.st_mx[col]);
+  const float mn = __uint_as_float(A.st_mn[col]);
+  const uint32_t q = A.st_q[col];
+  const unsigned long long dsum = A.st_dsum[col];
+  const int ell = A.ell[col] + 1;
+  const float gmax = mx;
+  const float gmin = q < B.n ? 0.0f : mn;
+  const uint32_t s = B.src0 + j;
+  const uint2 r = A.rowd[B.voff + s];
+  const double dsd = static_cast<double>(r.y);
+  const double eps = A.g_eps[B.gi], logmd = A.g_logmd[B.gi];
+  const double dse_d = dsd * eps;
+  const double dse2 = dse_d * dse_d;
+  const float* X = xn + B.xoff;
+  const double ps = static_cast<double>(X[static_cast<uint64_t>(s) * kBlkCols + j]);
+  double rw = 0.0;
+  for (uint32_t p = r.x + lane_id(); p < r.x + r.y; p += 64) {
+    const uint32_t v = A.adj[p] - B.voff;
+    A.preds_d[p] += (ps - static_cast<double>(X[static_cast<uint64_t>(v) * kBlkCols + j])) / dsd;
+    const int rest = A.taus[p] - ell;
+    if (rest > 0) {
+      const double chi = 2.0 * rest * (static_cast<double>(gmax) - gmin);
+      rw += fmax(ceil(8 * (chi * chi) * logmd / dse2), 1.0);
+    }
+  }
+  rw = wave_sum(rw);
+  if (lane_id() == 0) {
+    A.ell[col] = ell;
+    A.gmax[col] = gmax;
+    A.gmin[col] = gmin;
+    A.qsize[col] = q;
+    if (static_cast<double>(dsum) >= 25 * rw) A.active[col] = 0;
+    atomicMax(&A.last[b], static_cast<uint32_t>(ell));
+    A.st_mx[col] = 0u;
+    A.st_mn[col] = kInfBits;
+    A.st_q[col] = 0u;
+    A.st_dsum[col] = 0ull;
+  }
+}
+
+// Final block state (the buffer its last round wrote) -> one contiguous
+// vector per source in the other buffer.  Also flags a source still active
+// after the last launched round (the round bound was wrong).
+__global__ void __launch_bounds__(256) blk_transpose_kernel(BlockArgs A, uint32_t b0, uint32_t b1) {
+  __shared__ float tile[kBlkRows][kBlkCols + 1];
+  const uint32_t t = blockIdx.x;
+  const uint32_t b = find_block(A.blocks, b0, b1, t);
+  const BlockInfo B = A.blocks[b];
+  const uint32_t row0 = (t - B.tile0) * kBlkRows;
+  const uint32_t L = A.last[b];
+  const float* F = ((L & 1u) ? A.xb : A.xa) + B.xoff;
+  float* O = ((L & 1u) ? A.xa : A.xb) + B.xoff;
+  const uint32_t lane = lane_id(), wv = wave_id();
+  for (uint32_t r = wv; r < kBlkRows; r += 4) {
+    const uint32_t u = row0 + r;
+    tile[r][lane] = u < B.n ? F[static_cast<uint64_t>(u) * kBlkCols + lane] : 0.0f;
+  }
+  __syncthreads();
+  const uint32_t u = row0 + lane;
+  for (uint32_t jj = wv; jj < B.cnt; jj += 4)
+    if (u < B.n) O[static_cast<uint64_t>(jj) * B.n + u] = tile[lane][jj];
+  if (row0 == 0 && wv == 0 && lane < B.cnt && A.active[b * kBlkCols + lane] != 0)
+    atomicOr(A.status, 2u);
+}
+
+// ---- post: top-gamma bound, chi, walk counts, 2-way walks -------------------
+
+struct PostShared {
+  float rf[kPW];
+  double rd[kPW];
+  uint32_t ru[kPW];
+  uint32_t heavy[kPB];
+  uint32_t n_heavy;
+  uint32_t k;
+};
+
+// One bouquet of the 2-way walk for lane b of an 8-lane group; returns the
+// lane's float sum over the bouquet's hops.  The group's 8 lanes run in
+// lockstep (same position, bouquet and hop counts).
+__device__ __forceinline__ float walk2_bouquet(const BlockArgs& A, const float* pv, uint32_t off,
+                                               uint32_t s, uint32_t v, uint32_t j, uint32_t b,
+                                               uint32_t step, uint32_t len, uint32_t giw) {
+  uint32_t c1 = s, c2 = v;
+  uint32_t s1 = (j * 8u + b + 1u) * step;
+  uint32_t s2 = s1, st = s1;
+  bool fb = false;
+  float x = 0.0f;
+  for (uint32_t h = 0; h < len; ++h) {
+    const uint64_t z = __ballot(!fb && (s1 == 0u || s2 == 0u));
+    fb = fb || (((z >> (giw * 8u)) & 0xFFull) != 0ull);
+    const uint2 d1 = A.rowd[c1];
+    const uint2 d2 = A.rowd[c2];
+    uint32_t k1, k2;
+    if (!fb) {
+      k1 = __umulhi(s1 ^ fmix32(c1 - off), d1.y);
+      k2 = __umulhi(s2 ^ fmix32(c2 - off), d2.y);
+      s1 *= d1.y;
+      s2 *= d2.y;
+    } else {
+      const uint32_t ra = splitmix32(st);
+      const uint32_t rb = splitmix32(st);
+      k1 = ra % d1.y;
+      k2 = rb % d2.y;
+    }
+    c1 = A.adj[d1.x + k1];
+    c2 = A.adj[d2.x + k2];
+    x += pv[c1 - off] - pv[c2 - off];
+  }
+  return x;
+}
+
+// FP64 sum of the 8 lanes of a group, lanes 0..7 in order.
+__device__ __forceinline__ double group_sum8(float x, uint32_t grp_base) {
+  double s = 0.0;
+#pragma unroll
+  for (uint32_t k = 0; k < 8; ++k) s += static_cast<double>(__shfl(x, static_cast<int>(grp_base + k)));
+  return s;
+}
+
+// gamma-th largest value of the frontier (pv > 0), counting ties; -inf when
+// the frontier is empty.  cand: kPB * kTopK floats of LDS.
+__device__ float top_gamma_value(const float* pv, uint32_t n, uint32_t rg, float* cand,
+                                 PostShared& sh) {
+  float thr = INFINITY, vstar = -INFINITY;
+  uint32_t above = 0;
+  if (rg <= kTopK) {
+    float top[kTopK];
+#pragma unroll
+    for (uint32_t i = 0; i < kTopK; ++i) top[i] = -INFINITY;
+    for (uint32_t u = threadIdx.x; u < n; u += kPB) {
+      const float x = pv[u];
+      if (x > 0.0f && x > top[kTopK - 1]) {
+#pragma unroll
+        for (uint32_t i = kTopK - 1; i > 0; --i)
+          top[i] = x > top[i - 1] ? top[i - 1] : (x > top[i] ? x : top[i]);
+        top[0] = x > top[0] ? x : top[0];
+      }
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < kTopK; ++i) cand[threadIdx.x * kTopK + i] = top[i];
+    __syncthreads();
+    for (uint32_t it = 0; it <= rg; ++it) {
+      float m1 = -INFINITY;
+      for (uint32_t i = threadIdx.x; i < kPB * kTopK; i += kPB) {
+        const float x = cand[i];
+        if (x > 0.0f && x < thr) m1 = fmaxf(m1, x);
+      }
+      m1 = block_reduce<kPW>(m1, sh.rf, OpMaxF{});
+      uint32_t c = 0;
+      for (uint32_t i = threadIdx.x; i < kPB * kTopK; i += kPB) c += (cand[i] > 0.0f && cand[i] == m1);
+      c = block_reduce<kPW>(c, sh.ru, OpSum{});
+      vstar = m1;
+      if (above + c >= rg || c == 0) break;
+      above += c;
+      thr = m1;
+    }
+    return vstar;
+  }
+  for (uint32_t it = 0; it <= rg; ++it) {  // gamma > 16: threshold passes over pv
+    float m1 = -INFINITY;
+    for (uint32_t u = threadIdx.x; u < n; u += kPB) {
+      const float x = pv[u];
+      if (x > 0.0f && x < thr) m1 = fmaxf(m1, x);
+    }
+    m1 = block_reduce<kPW>(m1, sh.rf, OpMaxF{});
+    uint32_t c = 0;
+    for (uint32_t u = threadIdx.x; u < n; u += kPB) c += (pv[u] > 0.0f && pv[u] == m1);
+    c = block_reduce<kPW>(c, sh.ru, OpSum{});
+    vstar = m1;
+    if (above + c >= rg || c == 0) break;
+    above += c;
+    thr = m1;
+  }
+  return vstar;
+}
+
+__device__ void post_source(const BlockArgs& A, const BlockInfo& B, uint32_t col, uint32_t s,
+                            const float* pv, float* cand, PostShared& sh) {
+  const uint32_t n = B.n, off = B.voff;
+  const uint32_t src = off + s;
+  const double eps = A.g_eps[B.gi], logmd = A.g_logmd[B.gi];
+  const int32_t gamma = A.g_gamma[B.gi];
+  const int ell = A.ell[col];
+  const float gmax = A.gmax[col], gmin = A.gmin[col];
+  const uint32_t Q = A.qsize[col];
+  const uint2 rs = A.rowd[src];
+  const uint32_t r0 = rs.x, r1 = rs.x + rs.y;
+  const double dsd = static_cast<double>(rs.y);
+  const double dse_d = dsd * eps;
+
+  // ---- top-gamma bound (aesc.h:414-442)
+  const uint32_t rg = min(static_cast<uint32_t>(gamma), Q);
+  const float vstar = top_gamma_value(pv, n, rg, cand, sh);
+  const float gamma_max = gamma < static_cast<int>(Q) ? fminf(1.0f, vstar) : 0.0f;
+  float emax = gmax + gamma_max;
+  {
+    float best = -INFINITY;
+    for (uint32_t base = 0; base < n; base += kPB) {
+      if (threadIdx.x == 0) sh.n_heavy = 0;
+      __syncthreads();
+      const uint32_t u = base + threadIdx.x;
+      if (u < n) {
+        const float pu = pv[u];
+        if (pu > 0.0f && pu >= vstar) {
+          const uint2 ru = A.rowd[off + u];
+          if (ru.y <= A.light) {
+            for (uint32_t k = 0; k < ru.y; ++k) {
+              const float pw = pv[A.adj[ru.x + k] - off];
+              if (pw > 0.0f && pw >= vstar) best = fmaxf(best, pu + pw);
+            }
+          } else {
+            sh.heavy[atomicAdd(&sh.n_heavy, 1u)] = u;
+          }
+        }
+      }
+      __syncthreads();
+      const uint32_t nh = sh.n_heavy;
+      for (uint32_t h = wave_id(); h < nh; h += kPW) {
+        const uint32_t hu = sh.heavy[h];
+        const uint2 ru = A.rowd[off + hu];
+        const float pu = pv[hu];
+        for (uint32_t k = lane_id(); k < ru.y; k += 64) {
+          const float pw = pv[A.adj[ru.x + k] - off];
+          if (pw > 0.0f && pw >= vstar) best = fmaxf(best, pu + pw);
+        }
+      }
+      __syncthreads();
+    }
+    best = block_reduce<kPW>(best, sh.rf, OpMaxF{});
+    emax = fmaxf(emax, best);
+  }
+  if (gamma == 1) emax = 2 * gmax;
+
+  // ---- chi and walk counts, one wave per CSR position (aesc.h:392-412, 508-521)
+  float smx = 0.0f, smn = FLT_MAX;
+  for (uint32_t p = r0 + threadIdx.x; p < r1; p += kPB) {
+    const float x = pv[A.adj[p] - off];
+    smx = fmaxf(smx, x);
+    smn = fminf(smn, x);
+  }
+  smx = block_reduce<kPW>(smx, sh.rf, OpMaxF{});
+  smn = block_reduce<kPW>(smn, sh.rf, OpMinF{});
+  const float dse = static_cast<float>(dse_d);
+  for (uint32_t p = r0 + wave_id(); p < r1; p += kPW) {
+    const int rest = A.taus[p] - ell;
+    if (rest <= 0) {
+      if (lane_id() == 0) A.nr_pos[p] = 0;
+      continue;
+    }
+    const uint2 rv = A.rowd[A.adj[p]];
+    float tmx = 0.0f, tmn = FLT_MAX;
+    for (uint32_t k = lane_id(); k < rv.y; k += 64) {
+      const float x = pv[A.adj[rv.x + k] - off];
+      tmx = fmaxf(tmx, x);
+      tmn = fminf(tmn, x);
+    }
+    tmx = wave_max(tmx);
+    tmn = wave_min(tmn);
+    if (lane_id() == 0) {
+      const int len = rest;
+      const float chi = gmax + (smx + tmx) / 2.0 + (len - 1) * emax - smn - tmn -
+                        2 * (len - 1) * gmin;
+      const double v = fmax(ceil(8 * (chi * chi) * logmd / (dse * dse)), 1.0);
+      uint64_t nr = 0;
+      if (!(v < kMaxWalksFast)) {
+        atomicOr(A.status, 1u);
+      } else {
+        nr = static_cast<uint64_t>(v);
+        nr += 8 - nr % 8;
+      }
+      A.nr_pos[p] = nr;
+    }
+  }
+  __syncthreads();
+
+  // ---- 2-way walks.  Positions with at most small_bq bouquets: one 8-lane
+  // group each (32 groups side by side); larger ones: the whole workgroup.
+  const uint32_t g = threadIdx.x >> 3, b = threadIdx.x & 7u;
+  const uint32_t giw = g & 7u, gbase = (threadIdx.x & 63u) & ~7u;
+  const uint64_t small_bq = A.small_bq;
+  for (uint32_t p = r0 + g; p < r1; p += kPB / 8) {
+    const int rest = A.taus[p] - ell;
+    const uint64_t nr = rest > 0 ? A.nr_pos[p] : 0;
+    if (nr == 0 || nr / 8 > small_bq) continue;
+    const uint32_t v = A.adj[p];
+    const uint32_t step = static_cast<uint32_t>(0xFFFFFFFFull / nr);
+    double acc = 0.0;
+    for (uint32_t jb = 0; jb < nr / 8; ++jb)
+      acc += group_sum8(walk2_bouquet(A, pv, off, src, v, jb, b, step, rest, giw), gbase);
+    if (b == 0) {
+      A.preds_d[p] += acc / (dsd * static_cast<double>(nr));
+      if (A.tr_x) A.tr_x[p] = static_cast<float>(acc);
+    }
+  }
+  __syncthreads();
+  for (uint32_t p = r0; p < r1; ++p) {
+    const int rest = A.taus[p] - ell;
+    const uint64_t nr = rest > 0 ? A.nr_pos[p] : 0;
+    if (nr / 8 <= small_bq) continue;
+    const uint32_t v = A.adj[p];
+    const uint32_t step = static_cast<uint32_t>(0xFFFFFFFFull / nr);
+    const uint64_t nbq = nr / 8;
+    double acc = 0.0;
+    for (uint64_t jb = g; jb < nbq; jb += kPB / 8)
+      acc += group_sum8(walk2_bouquet(A, pv, off, src, v, static_cast<uint32_t>(jb), b, step,
+                                      rest, giw),
+                        gbase);
+    double mine = (b == 0) ? acc : 0.0;  // one contribution per group, groups in order
+    mine = block_reduce<kPW>(mine, sh.rd, OpSum{});
+    if (threadIdx.x == 0) {
+      A.preds_d[p] += mine / (dsd * static_cast<double>(nr));
+      if (A.tr_x) A.tr_x[p] = static_cast<float>(mine);
+    }
+  }
+  if (threadIdx.x == 0) {
+    if (A.tr_ell) A.tr_ell[src] = ell;
+    if (A.tr_emax) A.tr_emax[src] = emax;
+    if (A.tr_gmax) A.tr_gmax[src] = gmax;
+    if (A.tr_gmin) A.tr_gmin[src] = gmin;
+    if (A.tr_qsize) A.tr_qsize[src] = Q;
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kPB) blk_post_kernel(BlockArgs A, uint32_t b0, uint32_t b1) {
+  extern __shared__ float lds_pv[];
+  __shared__ float cand[kPB * kTopK];
+  __shared__ PostShared sh;
+  const uint32_t total = (b1 - b0) * kBlkCols;
+  for (;;) {
+    if (threadIdx.x == 0) sh.k = atomicAdd(A.next_src, 1u);
+    __syncthreads();
+    const uint32_t k = sh.k;
+    __syncthreads();
+    if (k >= total) break;
+    const uint32_t b = b0 + k / kBlkCols, j = k % kBlkCols;
+    const BlockInfo B = A.blocks[b];
+    if (j >= B.cnt) continue;  // uniform per workgroup
+    const uint32_t L = A.last[b];
+    const float* O = ((L & 1u) ? A.xa : A.xb) + B.xoff + static_cast<uint64_t>(j) * B.n;
+    const float* pv = O;
+    if (B.n <= kPostLds) {
+      for (uint32_t u = threadIdx.x; u < B.n; u += kPB) lds_pv[u] = O[u];
+      __syncthreads();
+      pv = lds_pv;
+    }
+    post_source(A, B, b * kBlkCols + j, B.src0 + j, pv, cand, sh);
+  }
+}
+
+}  // namespace
+
+int block_post_per_cu() {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, reinterpret_cast<const void*>(&blk_post_kernel), kPB,
+          kPostLds * sizeof(float)) != hipSuccess)
+    per_cu = 1;
+  return std::max(1, per_cu);
+}
+
+int launch_block_subset(tgtp_ctx* ctx, const BlockArgs& A, const BlockSubset& S,
+                        uint32_t post_slots) {
+  hipStream_t s = ctx->stream;
+  const uint32_t nb = S.b1 - S.b0;
+  const uint32_t col_blocks = (nb * kBlkCols + 3) / 4;
+  const uint64_t zb = std::min<uint64_t>(16384, (S.floats + 255) / 256 + 1);
+  blk_zero_kernel<<<static_cast<uint32_t>(zb), 256, 0, s>>>(A.xa, S.floats);
+  blk_init_kernel<<<col_blocks, 256, 0, s>>>(A, S.b0, S.b1);
+  for (uint32_t r = 1; r <= S.rounds; ++r) {
+    const float* cur = (r & 1u) ? A.xa : A.xb;
+    float* nxt = (r & 1u) ? A.xb : A.xa;
+    blk_round_kernel<<<S.tiles, 256, 0, s>>>(A, S.b0, S.b1, cur, nxt);
+    blk_stop_kernel<<<col_blocks, 256, 0, s>>>(A, S.b0, S.b1, nxt);
+  }
+  blk_transpose_kernel<<<S.tiles, 256, 0, s>>>(A, S.b0, S.b1);
+  blk_zero_kernel<<<1, 256, 0, s>>>(reinterpret_cast<float*>(A.next_src), 1);
+  blk_post_kernel<<<std::max(1u, std::min(post_slots, nb * kBlkCols)), kPB,
+                    kPostLds * sizeof(float), s>>>(A, S.b0, S.b1);
+  TGTP_HIP_CHECK(ctx, hipGetLastError());
+  return TGTP_OK;
+}

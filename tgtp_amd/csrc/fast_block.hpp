@@ -1,0 +1,82 @@
+// fast_block.hpp -- launch interface of the blocked FAST_FP32 / FAST_BF16
+// per-source pipeline (fast_block.hip).
+//
+// Sources are processed in blocks of 64 (one block = up to 64 sources of one
+// graph).  A block's diffusion state is a dense n x 64 float matrix
+// (vertex-major: the 64 sources' values of one vertex are one 256-byte row),
+// and one diffusion round of all 64 sources is one sparse-times-dense
+// product with the graph's random-walk matrix.  Blocks are grouped into
+// subsets whose state fits the plan's buffers; a subset runs
+//   zero + init, R x (round, stop), transpose, post
+// where R bounds the rounds any source of the subset can take (the largest
+// tau of its rows), the stop kernel applies the reference's cost rule per
+// source, and post does the top-gamma bound, chi, walk counts and 2-way walks
+// per source on its transposed (contiguous) vector.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#include "common.hpp"
+
+constexpr uint32_t kBlkCols = 64;  // sources per block (one per lane)
+constexpr uint32_t kBlkRows = 64;  // vertices per round tile
+
+struct BlockInfo {
+  uint32_t voff;   // first vertex of the block's graph in the union
+  uint32_t n;      // vertices of that graph
+  uint32_t src0;   // first source of the block (graph-local id)
+  uint32_t cnt;    // sources in the block (<= 64)
+  uint32_t gi;     // graph index
+  uint32_t tile0;  // round tiles of the subset before this block
+  uint64_t xoff;   // float offset of the block's n x 64 state in the subset buffers
+};
+
+struct BlockArgs {
+  const uint2* rowd;
+  const uint32_t* adj;
+  const float* degf;
+  const int32_t* taus;
+  double* preds_d;     // [m] FP64 row sums
+  uint64_t* nr_pos;    // [m]
+  uint32_t* status;    // bit 0: walk count beyond 2^36; bit 1: round bound exceeded
+  const double* g_eps;
+  const double* g_logmd;
+  const int32_t* g_gamma;
+  const BlockInfo* blocks;
+  float* xa;           // subset state buffers
+  float* xb;
+  // per source column, indexed block * 64 + lane
+  uint8_t* active;
+  int32_t* ell;
+  float* gmax;
+  float* gmin;
+  uint32_t* qsize;
+  uint32_t* st_mx;     // round statistics (float bits), reset by the stop kernel
+  uint32_t* st_mn;
+  uint32_t* st_q;
+  unsigned long long* st_dsum;
+  uint32_t* last;      // per block: last round any of its sources took
+  uint32_t* next_src;  // post kernel work counter
+  uint32_t light;      // degree up to which one thread scans a vertex's row
+  uint32_t small_bq;   // walk positions with at most this many bouquets: one 8-lane group
+  int32_t* tr_ell;
+  float* tr_emax;
+  float* tr_gmax;
+  float* tr_gmin;
+  uint32_t* tr_qsize;
+  float* tr_x;
+};
+
+struct BlockSubset {
+  uint32_t b0, b1;     // blocks [b0, b1)
+  uint32_t tiles;      // round tiles (kBlkRows vertices of one block each)
+  uint32_t rounds;     // rounds to launch (>= every source's stopping round)
+  uint64_t floats;     // state floats per buffer
+};
+
+// Enqueues the whole per-source phase for one subset on ctx->stream.
+int launch_block_subset(tgtp_ctx* ctx, const BlockArgs& A, const BlockSubset& S,
+                        uint32_t post_slots);
+// Resident workgroups per CU of the post kernel.
+int block_post_per_cu();

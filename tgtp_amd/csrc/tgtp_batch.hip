@@ -5,18 +5,23 @@
 // shifted per graph), one eigenvector array, per-graph tables.  A FAST plan
 // holds everything a run needs on the device -- per-graph scalars, the
 // padded 16-edge tiles of the tau projection with their column tables, the
-// work list of sources (largest graphs first, so the longest sources start
-// first) and all scratch -- so a run is four kernels on one stream:
-// clear, tau projection, per-source pipeline, symmetrization.  The batch
-// captures that sequence into a hipGraph once and replays it.
+// blocks of 64 sources (largest graphs first) grouped into subsets whose
+// n x 64 diffusion states fit the plan's two state buffers, the round bound
+// of every subset, and all scratch -- so a run is a fixed kernel sequence on
+// one stream: clear, tau projection, per subset (init, rounds, transpose,
+// post), symmetrization.  A single graph is a batch of one, so the plain and
+// batched forms compute the same scores bit for bit.  The batch captures its
+// sequence into a hipGraph once and replays it; many small graphs then share
+// each subset's launches instead of each paying its own.
 //
-// Autotuning (TGTP_BATCH_AUTOTUNE): coordinate descent over the light-vertex
-// degree threshold of the frontier kernels, the bouquet count below which a
-// walk position gets one 8-lane group instead of the whole workgroup, and the
-// number of resident workgroups, each candidate timed with HIP events on a
-// full run; the best is kept for the batch's captured graph.
+// Autotuning (TGTP_BATCH_AUTOTUNE): coordinate descent over the degree up to
+// which one thread scans a row in the post kernel, the bouquet count below
+// which a walk position gets one 8-lane group instead of the whole
+// workgroup, and the number of resident post workgroups, each candidate timed
+// with HIP events on a full run; the best is kept for the captured graph.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <numeric>
@@ -25,10 +30,16 @@
 #include "common.hpp"
 #include "host_tgtp.hpp"
 #include "taus_fast.hpp"
+#include "fast_block.hpp"
 #include "tgtp_fast.hpp"
 #include "tgtp_internal.hpp"
 
 namespace {
+
+int env_int_batch(const char* name, int dflt) {
+  const char* v = std::getenv(name);
+  return v && *v ? std::atoi(v) : dflt;
+}
 
 struct GraphSpec {        // host description of one graph of a plan
   uint32_t voff, n;       // vertex range in the union
@@ -59,18 +70,26 @@ struct FastPlan {
   float* d_wt = nullptr;
   __bf16* d_wtb = nullptr;
   __bf16* d_xb = nullptr;
-  uint32_t* d_work = nullptr;
-  uint32_t n_work = 0;
+  // source blocks (64 sources of one graph, largest graphs first) and the
+  // subsets whose n x 64 states share the two state buffers
+  BlockInfo* d_blocks = nullptr;
+  uint32_t n_blocks = 0;
+  uint64_t n_sources = 0;
+  std::vector<BlockSubset> subsets;
   // scratch
   int32_t* d_taus = nullptr;
   double* d_preds = nullptr;
   uint64_t* d_nr = nullptr;
-  uint32_t* d_q = nullptr;
-  float* d_pv = nullptr;
+  float* d_sa = nullptr;           // state buffers
+  float* d_sb = nullptr;
+  uint8_t* d_active = nullptr;
+  int32_t* d_ell = nullptr;
+  float* d_colf = nullptr;         // gmax, gmin per column
+  uint32_t* d_colu = nullptr;      // qsize, st_mx, st_mn, st_q per column; last per block
+  unsigned long long* d_dsum = nullptr;
   uint32_t* d_words = nullptr;
   float* d_out = nullptr;
-  uint32_t lds_n = 0, slot_n = 0, qcap = 0;
-  uint32_t max_slots = 0, slots = 0;
+  uint32_t max_slots = 0, slots = 0;  // resident workgroups of the post kernel
   // tuned parameters
   uint32_t light = 32, small_bq = 8;
   // captured run
@@ -83,8 +102,9 @@ void free_plan(FastPlan* P) {
   if (P->exec) (void)hipGraphExecDestroy(P->exec);
   if (P->graph) (void)hipGraphDestroy(P->graph);
   void* ptrs[] = {P->d_voff, P->d_eps, P->d_logmd, P->d_gamma, P->d_tgraphs, P->d_und,
-                  P->d_tile_graph, P->d_wt, P->d_wtb, P->d_xb, P->d_work, P->d_taus,
-                  P->d_preds, P->d_nr, P->d_q, P->d_pv, P->d_words, P->d_out};
+                  P->d_tile_graph, P->d_wt, P->d_wtb, P->d_xb, P->d_blocks, P->d_taus,
+                  P->d_preds, P->d_nr, P->d_sa, P->d_sb, P->d_active, P->d_ell, P->d_colf,
+                  P->d_colu, P->d_dsum, P->d_words, P->d_out};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete P;
@@ -109,6 +129,8 @@ int dev_upload(tgtp_ctx* ctx, T** p, const std::vector<T>& host, const char* wha
                                        hipMemcpyHostToDevice, ctx->stream));
   return TGTP_OK;
 }
+
+TauFastArgs plan_tau_args(const FastPlan* P, tgtp_dgraph* dg);
 
 // Builds the device tables and scratch of a FAST run over `dg` (a single
 // graph or a union) whose eigenvectors are already in dg->X.
@@ -230,67 +252,118 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
   if ((rc = dev_upload(ctx, &P->d_und, und, "tau tiles")) ||
       (rc = dev_upload(ctx, &P->d_tile_graph, tile_graph, "tile graphs")))
     return fail(rc);
-  // work list: sources of the largest graphs first
-  std::vector<uint32_t> order(G);
-  std::iota(order.begin(), order.end(), 0u);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](uint32_t a, uint32_t b) { return specs[a].m > specs[b].m; });
-  std::vector<uint32_t> work;
-  work.reserve(dg->n);
-  for (uint32_t k : order)
-    for (uint32_t v = 0; v < specs[k].n; ++v) work.push_back(specs[k].voff + v);
-  P->n_work = static_cast<uint32_t>(work.size());
-  if ((rc = dev_upload(ctx, &P->d_work, work, "work list"))) return fail(rc);
-  // per-source state: LDS for graphs that fit, HBM slots for the rest
-  for (const GraphSpec& S : specs) {
-    P->qcap = std::max(P->qcap, S.n);
-    if (fast_fits_lds(S.n))
-      P->lds_n = std::max(P->lds_n, S.n);
-    else
-      P->slot_n = std::max(P->slot_n, S.n);
-  }
-  const int per_cu = fast_slots_per_cu(P->lds_n);
-  P->max_slots = static_cast<uint32_t>(
-      std::min<uint64_t>(P->n_work, static_cast<uint64_t>(ctx->num_cus) * per_cu));
-  P->slots = P->max_slots;
-  if ((rc = dev_alloc(ctx, &P->d_q, static_cast<uint64_t>(P->max_slots) * 2 * P->qcap, "queues")))
-    return fail(rc);
-  if (P->slot_n &&
-      (rc = dev_alloc(ctx, &P->d_pv, P->max_slots * fast_slot_floats(P->slot_n), "source state")))
-    return fail(rc);
   if ((rc = dev_alloc(ctx, &P->d_taus, dg->m, "taus")) ||
       (rc = dev_alloc(ctx, &P->d_preds, dg->m, "preds")) ||
       (rc = dev_alloc(ctx, &P->d_nr, dg->m, "walk counts")) ||
       (rc = dev_alloc(ctx, &P->d_out, dg->m, "scores")) ||
       (rc = dev_alloc(ctx, &P->d_words, 16, "counters")))
     return fail(rc);
+  // taus once, to bound the rounds each block can take (a source stops by
+  // the round equal to the largest tau of its row, when no walk is left)
+  if ((rc = launch_fill_i32(ctx, P->d_taus, P->m, 0)) ||
+      (rc = launch_taus_fast(ctx, plan_tau_args(P, dg), mode == TGTP_MODE_FAST_BF16)))
+    return fail(rc);
+  std::vector<int32_t> taus(dg->m);
+  if (dg->m)
+    TGTP_HIP_CHECK(ctx, hipMemcpyAsync(taus.data(), P->d_taus, dg->m * 4, hipMemcpyDeviceToHost, s));
+  TGTP_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  // blocks: sources of the largest graphs first
+  std::vector<uint32_t> order(G);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t a, uint32_t b) { return specs[a].m > specs[b].m; });
+  std::vector<BlockInfo> blocks;
+  std::vector<uint32_t> bround;
+  for (uint32_t k : order) {
+    const GraphSpec& S = specs[k];
+    for (uint32_t s0 = 0; s0 < S.n; s0 += kBlkCols) {
+      BlockInfo B{};
+      B.voff = S.voff;
+      B.n = S.n;
+      B.src0 = s0;
+      B.cnt = std::min<uint32_t>(kBlkCols, S.n - s0);
+      B.gi = k;
+      int32_t tmax = 1;
+      for (uint32_t v = S.voff + s0; v < S.voff + s0 + B.cnt; ++v)
+        for (uint64_t p = idx[v]; p < idx[v + 1]; ++p) tmax = std::max(tmax, taus[p]);
+      blocks.push_back(B);
+      bround.push_back(static_cast<uint32_t>(tmax));
+    }
+  }
+  P->n_blocks = static_cast<uint32_t>(blocks.size());
+  P->n_sources = dg->n;
+  // subsets of blocks whose states fit one state buffer
+  const uint64_t budget =
+      static_cast<uint64_t>(std::max(1, env_int_batch("TGTP_FAST_STATE_MIB", 2048))) << 18;  // floats
+  uint64_t max_floats = 0;
+  for (uint32_t b0 = 0; b0 < P->n_blocks;) {
+    BlockSubset S{};
+    S.b0 = b0;
+    uint64_t fl = 0;
+    uint32_t tiles = 0, rounds = 1;
+    uint32_t b = b0;
+    while (b < P->n_blocks) {
+      const uint64_t need = static_cast<uint64_t>(blocks[b].n) * kBlkCols;
+      if (b > b0 && fl + need > budget) break;
+      blocks[b].xoff = fl;
+      blocks[b].tile0 = tiles;
+      fl += need;
+      tiles += (blocks[b].n + kBlkRows - 1) / kBlkRows;
+      rounds = std::max(rounds, bround[b]);
+      ++b;
+    }
+    S.b1 = b;
+    S.tiles = tiles;
+    S.rounds = rounds;
+    S.floats = fl;
+    max_floats = std::max(max_floats, fl);
+    P->subsets.push_back(S);
+    b0 = b;
+  }
+  const uint64_t cols = static_cast<uint64_t>(P->n_blocks) * kBlkCols;
+  if ((rc = dev_upload(ctx, &P->d_blocks, blocks, "source blocks")) ||
+      (rc = dev_alloc(ctx, &P->d_sa, max_floats, "diffusion state")) ||
+      (rc = dev_alloc(ctx, &P->d_sb, max_floats, "diffusion state")) ||
+      (rc = dev_alloc(ctx, &P->d_active, cols, "column flags")) ||
+      (rc = dev_alloc(ctx, &P->d_ell, cols, "column rounds")) ||
+      (rc = dev_alloc(ctx, &P->d_colf, 2 * cols, "column bounds")) ||
+      (rc = dev_alloc(ctx, &P->d_colu, 4 * cols + P->n_blocks, "column statistics")) ||
+      (rc = dev_alloc(ctx, &P->d_dsum, cols, "column degree sums")))
+    return fail(rc);
+  P->max_slots = static_cast<uint32_t>(static_cast<uint64_t>(ctx->num_cus) * block_post_per_cu());
+  P->slots = P->max_slots;
   TGTP_HIP_CHECK(ctx, hipStreamSynchronize(s));
   *out = P;
   return TGTP_OK;
 }
 
-FastArgs plan_args(const FastPlan* P, tgtp_dgraph* dg) {
-  FastArgs A{};
+BlockArgs plan_args(const FastPlan* P, tgtp_dgraph* dg) {
+  BlockArgs A{};
+  const uint64_t cols = static_cast<uint64_t>(P->n_blocks) * kBlkCols;
   A.rowd = dg->rowd;
   A.adj = dg->adj;
   A.degf = dg->degf;
   A.taus = P->d_taus;
   A.preds_d = P->d_preds;
   A.nr_pos = P->d_nr;
-  A.qbuf = P->d_q;
-  A.pv_g = P->d_pv;
-  A.next_src = P->d_words;
   A.status = P->d_words + 2;
-  A.voff = P->d_voff;
-  A.n_graphs = P->n_graphs;
   A.g_eps = P->d_eps;
   A.g_logmd = P->d_logmd;
   A.g_gamma = P->d_gamma;
-  A.work = P->d_work;
-  A.n_work = P->n_work;
-  A.lds_n = P->lds_n;
-  A.slot_n = P->slot_n;
-  A.qcap = P->qcap;
+  A.blocks = P->d_blocks;
+  A.xa = P->d_sa;
+  A.xb = P->d_sb;
+  A.active = P->d_active;
+  A.ell = P->d_ell;
+  A.gmax = P->d_colf;
+  A.gmin = P->d_colf + cols;
+  A.qsize = P->d_colu;
+  A.st_mx = P->d_colu + cols;
+  A.st_mn = P->d_colu + 2 * cols;
+  A.st_q = P->d_colu + 3 * cols;
+  A.last = P->d_colu + 4 * cols;
+  A.st_dsum = P->d_dsum;
+  A.next_src = P->d_words;
   A.light = P->light;
   A.small_bq = P->small_bq;
   return A;
@@ -318,15 +391,18 @@ TauFastArgs plan_tau_args(const FastPlan* P, tgtp_dgraph* dg) {
 // The run itself: device work only (capturable).  Events 5 and 6 (when
 // record_events) split tau / per-source / symmetrization.
 int enqueue_run(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_of,
-                bool record_events, const FastArgs* override_args) {
+                bool record_events, const BlockArgs* override_args) {
   int rc;
   if ((rc = launch_fill_words(ctx, P->d_words, 16)) ||
       (rc = launch_fill_i32(ctx, P->d_taus, P->m, 0)) ||
       (rc = launch_taus_fast(ctx, plan_tau_args(P, dg), P->mode == TGTP_MODE_FAST_BF16)))
     return rc;
   if (record_events) TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
-  const FastArgs A = override_args ? *override_args : plan_args(P, dg);
-  if ((rc = launch_fast_sources(ctx, A, P->slots))) return rc;
+  BlockArgs A = override_args ? *override_args : plan_args(P, dg);
+  A.light = P->light;
+  A.small_bq = P->small_bq;
+  for (const BlockSubset& S : P->subsets)
+    if ((rc = launch_block_subset(ctx, A, S, P->slots))) return rc;
   if (record_events) TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
   return launch_symmetrize_d(ctx, P->d_preds, src_of, P->m, P->d_out);
 }
@@ -394,6 +470,8 @@ int check_status(tgtp_ctx* ctx, FastPlan* P) {
   if (words[2] & 1u)
     return set_error(ctx, TGTP_E_UNSUPPORTED,
                      "a walk count exceeded 2^36 walks per edge; scores are not computed");
+  if (words[2] & 2u)
+    return set_error(ctx, TGTP_E_HIP, "a source diffused beyond its round bound");
   return TGTP_OK;
 }
 
@@ -440,7 +518,7 @@ int tgtp_run_score_fast(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e,
   if (rc) return rc;
   hipStream_t s = ctx->stream;
   float* tr_buf = nullptr;
-  FastArgs A = plan_args(P, dg);
+  BlockArgs A = plan_args(P, dg);
   if (tr) {
     const uint64_t n = dg->n, m = dg->m;
     tr_buf = static_cast<float*>(ctx_buffer(ctx, kBufTrace, (5 * n + m) * 4 + 64));
@@ -672,7 +750,7 @@ extern "C" int tgtp_batch_run(tgtp_ctx* ctx, tgtp_batch* b, float* const* scores
     TGTP_HIP_CHECK(ctx, hipEventElapsedTime(&a, ctx->ev[4], ctx->ev[7]));
     stats->kernel_ms = a;
     stats->e2e_ms = tgtp_now_ms() - t0;
-    stats->work_items = b->plan->n_work;
+    stats->work_items = b->plan->n_sources;
     stats->taus_ms = b->tune_ms;
   }
   return TGTP_OK;

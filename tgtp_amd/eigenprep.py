@@ -144,6 +144,153 @@ def block_krylov_eigs(P, omega: int, tol: float = 1e-9, block: int = 64, basis: 
     raise RuntimeError(f"block Krylov eigensolver: no convergence in {max_cycles} cycles")
 
 
+def _cholqr2(Z):
+    """Orthonormal basis of the columns of each Z[i] (k, n, b) by two
+    Cholesky-QR passes (GEMM-shaped, unlike Householder QR).  Returns (Q,
+    ok[k]); ok is False where a Gram matrix was not positive definite."""
+    import torch
+
+    ok = torch.ones(Z.shape[0], dtype=torch.bool, device=Z.device)
+    for _ in range(2):
+        G = Z.mT @ Z
+        L, info = torch.linalg.cholesky_ex(0.5 * (G + G.mT))
+        bad = info != 0
+        ok &= ~bad
+        if bool(bad.any()):
+            eye = torch.eye(G.shape[-1], dtype=G.dtype, device=G.device)
+            L = torch.where(bad[:, None, None], eye, L)
+        # Z L^-T through the small explicit inverse: one batched GEMM (a
+        # batched triangular solve on the tall side needs a large workspace)
+        eye = torch.eye(G.shape[-1], dtype=G.dtype, device=G.device).expand_as(L)
+        Linv_t = torch.linalg.solve_triangular(L, eye, upper=False).mT.contiguous()
+        Z = Z @ Linv_t
+    return Z, ok
+
+
+def batched_block_krylov_eigs(P, k: int, n_pad: int, n_real, omega: int, tol: float = 1e-9,
+                              block: int = 64, basis: int = 768, keep: int | None = None,
+                              seed: int = 1, max_cycles: int = 200):
+    """block_krylov_eigs for k graphs at once.  P is the block-diagonal
+    union (torch CSR, float64) of the k normalized adjacencies, each padded
+    to n_pad vertices with empty rows (eigenvalue 0, never among the wanted
+    ones); n_real[i] is graph i's vertex count.  Every product on the basis
+    is one batched GEMM over the k graphs and one SpMM with the union, so a
+    size class of graphs costs about as many launches as one graph.
+    Returns (theta[k, omega], Y[k, n_pad, omega])."""
+    import torch
+
+    dev = P.device
+    keep = keep or omega + max(omega // 2, 64)
+    basis = min(basis, min(n_real))
+    block = min(block, max(1, basis - keep))
+    gen = np.random.default_rng(seed)
+    mask = torch.zeros((k, n_pad, 1), dtype=torch.float64, device=dev)
+    for i, ni in enumerate(n_real):
+        mask[i, :ni] = 1.0
+
+    def rand(cols):
+        return torch.from_numpy(gen.standard_normal((k, n_pad, cols))).to(dev) * mask
+
+    Q = torch.empty((k, n_pad, basis), dtype=torch.float64, device=dev)
+    W = torch.empty_like(Q)
+    out_th = torch.zeros((k, omega), dtype=torch.float64, device=dev)
+    out_Y = torch.zeros((k, n_pad, omega), dtype=torch.float64, device=dev)
+    done = torch.zeros(k, dtype=torch.bool, device=dev)
+
+    def orth(Z, c):
+        for _ in range(2):
+            if c:
+                Z = Z - Q[:, :, :c] @ (Q[:, :, :c].mT @ Z)
+        Z, ok = _cholqr2(Z)
+        if not bool(ok.all()):
+            # an invariant subspace was reached: random directions instead
+            Z = torch.where(ok[:, None, None], Z, rand(Z.shape[-1]))
+            for _ in range(2):
+                if c:
+                    Z = Z - Q[:, :, :c] @ (Q[:, :, :c].mT @ Z)
+            Z, _ = _cholqr2(Z)
+        return Z
+
+    nxt = orth(rand(block), 0)
+    c = 0
+    for _cycle in range(max_cycles):
+        while c + block <= basis:
+            Q[:, :, c:c + block] = nxt
+            W[:, :, c:c + block] = (P @ nxt.reshape(k * n_pad, block)).reshape(k, n_pad, block)
+            c += block
+            nxt = orth(W[:, :, c - block:c].clone(), c)
+        H = Q[:, :, :c].mT @ W[:, :, :c]
+        H = 0.5 * (H + H.mT)
+        th, S = torch.linalg.eigh(H)
+        order = torch.argsort(-th.abs(), dim=1, stable=True)
+        th = torch.gather(th, 1, order)
+        S = torch.gather(S, 2, order[:, None, :keep].expand(-1, c, -1))
+        Y = Q[:, :, :c] @ S
+        PY = W[:, :, :c] @ S
+        res = torch.linalg.vector_norm(PY[:, :, :omega] - Y[:, :, :omega] * th[:, None, :omega],
+                                       dim=1).amax(dim=1)
+        newly = (res < tol) & ~done
+        if bool(newly.any()):
+            out_th[newly] = th[newly, :omega]
+            out_Y[newly] = Y[newly, :, :omega]
+            done |= newly
+        if bool(done.all()):
+            return out_th, out_Y
+        Q[:, :, :keep] = Y
+        W[:, :, :keep] = PY
+        c = keep
+        nxt = orth(nxt, c)
+    raise RuntimeError(f"batched block Krylov eigensolver: no convergence in {max_cycles} cycles")
+
+
+def compute_eigens_many(graphs, omega: int, device: str = "cuda:0", tol: float = 1e-9,
+                        seed: int = 1, max_pad: float = 1.5, mem_bytes: float = 8e9,
+                        progress=None):
+    """compute_eigens for many graphs on a torch device: graphs are grouped
+    into size classes (n within max_pad of the class's smallest, basis
+    memory within mem_bytes) and each class runs one batched block Krylov
+    solve on the union of its graphs.  Returns [(values, vectors)] in the
+    order of `graphs`."""
+    import scipy.sparse as sp
+    import torch
+
+    order = sorted(range(len(graphs)), key=lambda i: graphs[i].n)
+    out = [None] * len(graphs)
+    pos = 0
+    while pos < len(order):
+        n0 = graphs[order[pos]].n
+        cls = [order[pos]]
+        while pos + len(cls) < len(order):
+            nxt_i = order[pos + len(cls)]
+            n_pad = graphs[nxt_i].n
+            if n_pad > max_pad * n0 or (len(cls) + 1) * n_pad * 768 * 16 > mem_bytes:
+                break
+            cls.append(nxt_i)
+        pos += len(cls)
+        n_pad = max(graphs[i].n for i in cls)
+        mats, extras = [], []
+        for i in cls:
+            Pi, rs, mu = normalized_adjacency(graphs[i])
+            pad = n_pad - graphs[i].n
+            mats.append(sp.block_diag([Pi, sp.csr_matrix((pad, pad))]) if pad else Pi)
+            extras.append((rs, mu))
+        U = sp.block_diag(mats, format="csr")
+        th, Y = batched_block_krylov_eigs(_torch_csr(U, device), len(cls), n_pad,
+                                          [graphs[i].n for i in cls], omega, tol=tol, seed=seed)
+        th = th.cpu().numpy()
+        Y = Y.cpu().numpy()
+        for j, i in enumerate(cls):
+            rs, mu = extras[j]
+            ni = graphs[i].n
+            scale = np.where(rs > 0, np.sqrt(2.0 * mu) / np.sqrt(np.maximum(rs, 1e-300)), 0.0)
+            out[i] = (th[j].astype(np.float64), Y[j, :ni] * scale[:, None])
+        del U, Y
+        torch.cuda.empty_cache() if str(device).startswith("cuda") else None
+        if progress:
+            progress(pos, len(graphs))
+    return out
+
+
 def compute_eigens_device(g: Graph, omega: int, device: str = "cuda:0", tol: float = 1e-9,
                           seed: int = 1, dense_limit: int = 512):
     """compute_eigens with the eigensolver on a torch device (an MI355X for
@@ -161,7 +308,7 @@ def compute_eigens_device(g: Graph, omega: int, device: str = "cuda:0", tol: flo
         order = torch.argsort(-w.abs(), stable=True)[:omega]
         w, V = w[order], V[:, order]
     else:
-        w, V = block_krylov_eigs(_torch_csr(P, device), omega, tol=tol, seed=seed)
+        return compute_eigens_many([g], omega, device=device, tol=tol, seed=seed)[0]
     w = w.cpu().numpy()
     V = V.cpu().numpy()
     scale = np.where(rs > 0, np.sqrt(2.0 * m_unique) / np.sqrt(np.maximum(rs, 1e-300)), 0.0)
