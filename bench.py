@@ -150,6 +150,12 @@ def bench_tgtp(args):
     return out
 
 
+def _timed(fn) -> float:
+    t1 = time.perf_counter()
+    fn()
+    return time.perf_counter() - t1
+
+
 def error_stats(scores, exact, eps, pos=None):
     import numpy as np
 
@@ -161,33 +167,42 @@ def error_stats(scores, exact, eps, pos=None):
             "within_eps": bool(err.max() <= eps), "edges_checked": int(err.shape[0])}
 
 
-def cpu_tgtp_baseline(graphs, eigens, eps, gpu_faithful=None):
+def cpu_tgtp_baseline(graphs, eigens, eps, gpu_faithful=None, sampled=None, reps=3):
     """The oracle's tgtp (C++ OpenMP over sources, the reference's
-    algorithm) on the host cores, min of 3 per graph; when the GPU faithful
-    scores are given, whether they equal the oracle's bit for bit."""
+    algorithm) on the host cores, min of `reps` per graph; when the GPU
+    faithful scores are given, whether they equal the oracle's bit for bit;
+    when (positions, exact) per graph are given, the reference algorithm's
+    own error on those edges."""
     import numpy as np
 
     import oracle
 
     threads = host_threads()
     total_m, total_s, equal = 0, 0.0, True
+    est, ex = [], []
     for k, (g, (lam, X)) in enumerate(zip(graphs, eigens)):
         best = None
-        for _ in range(3):
+        for _ in range(reps):
             t1 = time.perf_counter()
             ref = oracle.tgtp(g.idx, g.adj, lam, X, eps, 10, threads=threads)
             dt = time.perf_counter() - t1
             best = dt if best is None else min(best, dt)
         if gpu_faithful is not None and gpu_faithful[k] is not None:
             equal &= bool(np.array_equal(ref.view(np.uint32), gpu_faithful[k].view(np.uint32)))
+        if sampled is not None:
+            est.append(ref[sampled[k][0]])
+            ex.append(sampled[k][1])
         total_m += g.m
         total_s += best
     out = {"edges_per_s": total_m / total_s, "seconds": total_s, "cores": threads,
            "kind": "port",
-           "sample": f"{len(graphs)} graph(s), {total_m} CSR entries, min of 3 per graph "
+           "sample": f"{len(graphs)} graph(s), {total_m} CSR entries, min of {reps} per graph "
                      "(oracle C++ OpenMP, schedule(dynamic) over sources)"}
     if gpu_faithful is not None:
         out["gpu_faithful_bitwise_equal"] = equal
+    if sampled is not None:
+        out["reference_algorithm_vs_exact_fp64_sampled"] = error_stats(
+            np.concatenate(est), np.concatenate(ex), eps)
     return out
 
 
@@ -230,23 +245,25 @@ def bench_batch(args):
            "graphs": len(graphs), "sum_n": sum(g.n for g in graphs), "sum_m": total_m,
            "omega": 128, "gamma": 10, "eigen_prep_s": round(t_eig, 1)}
     reps = 2
+    fast_out = None
     with Context(0) as ctx:
         for eps in (0.05, 0.01):
             leg = {}
             for mode, name in ((MODE_FAST_FP32, "fast_fp32"), (MODE_FAST_BF16, "fast_bf16")):
-                plain_s, plain_out = 0.0, []
+                plain_s, plain_out, plain_each = 0.0, [], []
                 for gi, (g, e) in enumerate(zip(graphs, eig)):
                     if gi % 64 == 63:
                         log(f"[bench] batch eps={eps} {name}: plain {gi + 1}/{len(graphs)}")
                     dg = ctx.upload(g)
                     ctx.score(dg, *e, eps, 10, mode=mode)  # eigen upload + plan
                     best = None
-                    for _ in range(reps):
+                    for _ in range(1):
                         t1 = time.perf_counter()
                         s, _ = ctx.score(dg, *e, eps, 10, mode=mode)
                         dt = time.perf_counter() - t1
                         best = dt if best is None else min(best, dt)
                     plain_s += best
+                    plain_each.append(best)
                     plain_out.append(s)
                     dg.free()
                 t1 = time.perf_counter()
@@ -261,6 +278,8 @@ def bench_batch(args):
                     kms.append(st["kernel_ms"])
                 cfg = b.config()
                 b.close()
+                if mode == MODE_FAST_FP32:
+                    fast_out = outs
                 best = min(times)
                 bit_equal = all(np.array_equal(a.view(np.uint32), p.view(np.uint32))
                                 for a, p in zip(outs, plain_out))
@@ -277,12 +296,38 @@ def bench_batch(args):
                     f"({best * 1e3:.1f} ms), plain {total_m / plain_s:.3e} "
                     f"({plain_s * 1e3:.1f} ms), x{plain_s / best:.2f}, cfg {cfg}, "
                     f"max err {leg[name]['vs_exact_fp64_sampled']['max_abs']:.4f}")
+                if mode == MODE_FAST_FP32:
+                    # the batch's small graphs alone, where per-call costs dominate
+                    small = [i for i, g in enumerate(graphs) if g.n <= args.batch_small_n]
+                    if small:
+                        sm = sum(graphs[i].m for i in small)
+                        sp = sum(plain_each[i] for i in small)
+                        b = ctx.batch([graphs[i] for i in small], [eig[i] for i in small], eps,
+                                      10, mode=mode, autotune=True, graph=True)
+                        b.run()
+                        sb = min(_timed(b.run) for _ in range(reps))
+                        b.close()
+                        leg[name]["small_graphs"] = {
+                            "graphs": len(small), "max_n": args.batch_small_n, "sum_m": sm,
+                            "batched_edges_per_s": sm / sb, "plain_edges_per_s": sm / sp,
+                            "batched_over_plain": sp / sb}
+                        log(f"[bench] batch eps={eps} {name}: {len(small)} graphs with n <= "
+                            f"{args.batch_small_n}: batched {sm / sb:.3e} edges/s, plain "
+                            f"{sm / sp:.3e}, x{sp / sb:.2f}")
             if not args.no_cpu:
-                sub = list(range(0, len(graphs), max(1, len(graphs) // args.batch_cpu_graphs)))
-                leg["cpu_baseline"] = cpu_tgtp_baseline([graphs[i] for i in sub],
-                                                        [eig[i] for i in sub], eps)
-                leg["cpu_baseline"]["sample"] += f"; every {max(1, len(graphs) // args.batch_cpu_graphs)}th graph of the batch"
-                log(f"[bench] batch eps={eps} cpu {leg['cpu_baseline']['edges_per_s']:.3e} edges/s")
+                every = max(1, len(graphs) // args.batch_cpu_graphs)
+                sub = list(range(0, len(graphs), every))
+                leg["cpu_baseline"] = cpu_tgtp_baseline(
+                    [graphs[i] for i in sub], [eig[i] for i in sub], eps,
+                    sampled=[(samples[i], exact[i]) for i in sub], reps=1)
+                leg["cpu_baseline"]["sample"] += f"; every {every}th graph of the batch"
+                fe = np.concatenate([fast_out[i][samples[i]] for i in sub])
+                leg["fast_fp32_vs_exact_fp64_same_graphs"] = error_stats(
+                    fe, np.concatenate([exact[i] for i in sub]), eps)
+                log(f"[bench] batch eps={eps} cpu {leg['cpu_baseline']['edges_per_s']:.3e} edges/s; "
+                    f"max err on the cpu graphs: reference algorithm "
+                    f"{leg['cpu_baseline']['reference_algorithm_vs_exact_fp64_sampled']['max_abs']:.4f}, "
+                    f"fast_fp32 {leg['fast_fp32_vs_exact_fp64_same_graphs']['max_abs']:.4f}")
             out[f"eps_{eps}"] = leg
     return out
 
@@ -325,6 +370,7 @@ def main() -> int:
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-sample-edges", type=int, default=32)
     ap.add_argument("--batch-cpu-graphs", type=int, default=16)
+    ap.add_argument("--batch-small-n", type=int, default=8192)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "

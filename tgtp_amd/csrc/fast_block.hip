@@ -38,6 +38,15 @@ constexpr uint32_t kTopK = 16;         // single-pass top-gamma for gamma <= 16
 constexpr double kMaxWalksFast = 68719476736.0;
 constexpr uint32_t kInfBits = 0x7f800000u;
 
+// Tile of this workgroup.  Workgroups are dealt to the 8 XCDs round-robin
+// (blockIdx.x % 8); each XCD gets a contiguous run of tiles, so the blocks
+// whose state rows its L2 caches are few.  The grid is 8 * ceil(tiles / 8);
+// a workgroup past the last tile returns.
+__device__ __forceinline__ uint32_t xcd_tile() {
+  const uint32_t per = gridDim.x >> 3;
+  return (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+}
+
 // Last block in [b0, b1) whose first round tile is <= t.
 __device__ __forceinline__ uint32_t find_block(const BlockInfo* B, uint32_t b0, uint32_t b1,
                                                uint32_t t) {
@@ -77,54 +86,128 @@ __global__ void __launch_bounds__(256) blk_init_kernel(BlockArgs A, uint32_t b0,
   }
   if (j >= B.cnt) return;
   const uint32_t s = B.src0 + j;
-  if (lane_id() == 0) A.xa[B.xoff + static_cast<uint64_t>(s) * kBlkCols + j] = 1.0f;
   const uint2 r = A.rowd[B.voff + s];
   const double dsd = static_cast<double>(r.y);
-  for (uint32_t p = r.x + lane_id(); p < r.x + r.y; p += 64) A.preds_d[p] = 1.0 / dsd;
+  float* X1 = A.xb + B.xoff + j;
+  for (uint32_t p = r.x + lane_id(); p < r.x + r.y; p += 64) {
+    A.preds_d[p] = 1.0 / dsd;
+    // round 1 from e_s touches only N(s): count s's occurrences in each
+    // neighbour's row (= u's occurrences in s's row; integers, exact in any
+    // order), divided by deg(u) in blk_first_kernel
+    atomicAdd(&X1[static_cast<uint64_t>(A.adj[p] - B.voff) * kBlkCols], 1.0f);
+  }
+}
+
+// Round 1: pvec_1[u] = count(s in adj(u)) / deg(u) on N(s) (the value the
+// dense round computes: a sum of ones in adjacency order, then the same
+// division), with the round's frontier statistics.  The first occurrence of
+// u in s's sorted row divides; duplicates are skipped.
+__global__ void __launch_bounds__(256) blk_first_kernel(BlockArgs A, uint32_t b0, uint32_t b1) {
+  const uint32_t c = blockIdx.x * 4u + wave_id();
+  const uint32_t b = b0 + c / kBlkCols, j = c % kBlkCols;
+  if (b >= b1) return;
+  const BlockInfo B = A.blocks[b];
+  if (j >= B.cnt) return;
+  const uint32_t col = b * kBlkCols + j;
+  const uint32_t s = B.src0 + j;
+  const uint2 r = A.rowd[B.voff + s];
+  float* X1 = A.xb + B.xoff + j;
+  float mx = 0.0f, mn = INFINITY;
+  uint32_t q = 0;
+  unsigned long long ds = 0;
+  for (uint32_t p = r.x + lane_id(); p < r.x + r.y; p += 64) {
+    const uint32_t ug = A.adj[p];
+    if (p > r.x && A.adj[p - 1] == ug) continue;
+    float* x = &X1[static_cast<uint64_t>(ug - B.voff) * kBlkCols];
+    const float val = *x / A.degf[ug];
+    *x = val;
+    mx = fmaxf(mx, val);
+    mn = fminf(mn, val);
+    ++q;
+    ds += A.rowd[ug].y;
+  }
+  mx = wave_max(mx);
+  mn = wave_min(mn);
+  q = wave_sum(q);
+  ds = wave_sum(ds);
+  if (lane_id() == 0) {
+    A.st_mx[col] = __float_as_uint(mx);
+    A.st_mn[col] = __float_as_uint(mn);
+    A.st_q[col] = q;
+    A.st_dsum[col] = ds;
+  }
 }
 
 // One round for every tile (64 vertices of one block) of the subset: lane j
-// computes source j's new value at each vertex of the tile.
+// computes source j's new value at each vertex of the tile.  One workgroup
+// per tile (a persistent loop over tiles measured slower: the hardware
+// overlaps a finishing workgroup with the next one, a loop with a barrier
+// per tile does not).
 __global__ void __launch_bounds__(256)
-blk_round_kernel(BlockArgs A, uint32_t b0, uint32_t b1, const float* __restrict__ cur,
-                 float* __restrict__ nxt) {
+blk_round_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_tiles,
+                 const float* __restrict__ cur, float* __restrict__ nxt) {
   __shared__ float s_mx[4][kBlkCols];
   __shared__ float s_mn[4][kBlkCols];
   __shared__ uint32_t s_q[4][kBlkCols];
   __shared__ unsigned long long s_ds[4][kBlkCols];
-  const uint32_t t = blockIdx.x;
+  const uint32_t t = xcd_tile();
+  if (t >= n_tiles) return;  // uniform per workgroup
   const uint32_t b = find_block(A.blocks, b0, b1, t);
   const BlockInfo B = A.blocks[b];
   const uint32_t j = lane_id(), wv = wave_id();
   const uint32_t col = b * kBlkCols + j;
   const bool act = A.active[col] != 0;
+  // the same flags in every wave: a block whose sources all stopped is
+  // skipped by the whole workgroup
+  if (__ballot(act) == 0ull) return;
   float mx = 0.0f, mn = INFINITY;
   uint32_t q = 0;
   unsigned long long ds = 0;
-  if (__ballot(act) != 0ull) {  // a block whose sources all stopped is left alone
-    const uint32_t row0 = (t - B.tile0) * kBlkRows;
-    const uint32_t rend = min(row0 + kBlkRows, B.n);
-    const float* Xc = cur + B.xoff;
-    float* Xn = nxt + B.xoff;
-    for (uint32_t u = row0 + wv; u < rend; u += 4) {
-      const uint64_t ro = static_cast<uint64_t>(u) * kBlkCols + j;
-      if (act) {
-        const uint2 r = A.rowd[B.voff + u];
-        float sum = 0.0f;
-#pragma unroll 8
-        for (uint32_t k = 0; k < r.y; ++k)
-          sum += Xc[static_cast<uint64_t>(A.adj[r.x + k] - B.voff) * kBlkCols + j];
-        const float val = sum / A.degf[B.voff + u];
-        Xn[ro] = val;
-        if (val > 0.0f) {
-          mx = fmaxf(mx, val);
-          mn = fminf(mn, val);
-          ++q;
-          ds += r.y;
-        }
-      } else {
-        Xn[ro] = Xc[ro];
+  const uint32_t row0 = (t - B.tile0) * kBlkRows;
+  const uint32_t rend = min(row0 + kBlkRows, B.n);
+  const float* Xc = cur + B.xoff;
+  float* Xn = nxt + B.xoff;
+  // row descriptors and neighbour ids are the same for the whole wave:
+  // made uniform, they are scalar loads and each neighbour costs one
+  // 256-byte vector load of its state row
+  const uint32_t wu = __builtin_amdgcn_readfirstlane(wv);
+  for (uint32_t u = row0 + wu; u < rend; u += 4) {
+    const uint64_t ro = static_cast<uint64_t>(u) * kBlkCols + j;
+    if (act) {
+      const uint2 r = A.rowd[B.voff + u];
+      const uint32_t start = __builtin_amdgcn_readfirstlane(r.x);
+      const uint32_t deg = __builtin_amdgcn_readfirstlane(r.y);
+      const uint32_t* nb = A.adj + start;
+      const float* xj = Xc + j;
+      const uint32_t voff = B.voff;
+      float sum = 0.0f;
+      uint32_t k = 0;
+      // loads of a chunk are issued together, the adds stay in adjacency order
+      for (; k + 16 <= deg; k += 16) {
+        float v[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) v[c] = xj[static_cast<uint64_t>(nb[k + c] - voff) * kBlkCols];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) sum += v[c];
       }
+      for (; k + 4 <= deg; k += 4) {
+        float v[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[c] = xj[static_cast<uint64_t>(nb[k + c] - voff) * kBlkCols];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sum += v[c];
+      }
+      for (; k < deg; ++k) sum += xj[static_cast<uint64_t>(nb[k] - voff) * kBlkCols];
+      const float val = sum / A.degf[B.voff + u];
+      Xn[ro] = val;
+      if (val > 0.0f) {
+        mx = fmaxf(mx, val);
+        mn = fminf(mn, val);
+        ++q;
+        ds += r.y;
+      }
+    } else {
+      Xn[ro] = Xc[ro];
     }
   }
   s_mx[wv][j] = mx;
@@ -202,9 +285,11 @@ blk_stop_kernel(BlockArgs A, uint32_t b0, uint32_t b1, const float* __restrict__
 // Final block state (the buffer its last round wrote) -> one contiguous
 // vector per source in the other buffer.  Also flags a source still active
 // after the last launched round (the round bound was wrong).
-__global__ void __launch_bounds__(256) blk_transpose_kernel(BlockArgs A, uint32_t b0, uint32_t b1) {
+__global__ void __launch_bounds__(256)
+blk_transpose_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_tiles) {
   __shared__ float tile[kBlkRows][kBlkCols + 1];
-  const uint32_t t = blockIdx.x;
+  const uint32_t t = xcd_tile();
+  if (t >= n_tiles) return;  // uniform per workgroup
   const uint32_t b = find_block(A.blocks, b0, b1, t);
   const BlockInfo B = A.blocks[b];
   const uint32_t row0 = (t - B.tile0) * kBlkRows;
@@ -226,6 +311,8 @@ __global__ void __launch_bounds__(256) blk_transpose_kernel(BlockArgs A, uint32_
 
 // ---- post: top-gamma bound, chi, walk counts, 2-way walks -------------------
 
+constexpr uint32_t kCandCap = 1024;  // listed members of the top-gamma set
+
 struct PostShared {
   float rf[kPW];
   double rd[kPW];
@@ -233,6 +320,8 @@ struct PostShared {
   uint32_t heavy[kPB];
   uint32_t n_heavy;
   uint32_t k;
+  uint32_t cand[kCandCap];
+  uint32_t n_cand;
 };
 
 // One bouquet of the 2-way walk for lane b of an 8-lane group; returns the
@@ -355,8 +444,32 @@ __device__ void post_source(const BlockArgs& A, const BlockInfo& B, uint32_t col
   const float gamma_max = gamma < static_cast<int>(Q) ? fminf(1.0f, vstar) : 0.0f;
   float emax = gmax + gamma_max;
   {
+    // the top-gamma set with its ties: listed in one pass, then one wave
+    // per member scans its row; an overlong tie set takes the chunked scan
     float best = -INFINITY;
-    for (uint32_t base = 0; base < n; base += kPB) {
+    if (threadIdx.x == 0) sh.n_cand = 0;
+    __syncthreads();
+    for (uint32_t u = threadIdx.x; u < n; u += kPB) {
+      const float pu = pv[u];
+      if (pu > 0.0f && pu >= vstar) {
+        const uint32_t i = atomicAdd(&sh.n_cand, 1u);
+        if (i < kCandCap) sh.cand[i] = u;
+      }
+    }
+    __syncthreads();
+    const uint32_t nc = sh.n_cand;
+    if (nc <= kCandCap) {
+      for (uint32_t h = wave_id(); h < nc; h += kPW) {
+        const uint32_t hu = sh.cand[h];
+        const uint2 ru = A.rowd[off + hu];
+        const float pu = pv[hu];
+        for (uint32_t k = lane_id(); k < ru.y; k += 64) {
+          const float pw = pv[A.adj[ru.x + k] - off];
+          if (pw > 0.0f && pw >= vstar) best = fmaxf(best, pu + pw);
+        }
+      }
+    }
+    for (uint32_t base = 0; nc > kCandCap && base < n; base += kPB) {
       if (threadIdx.x == 0) sh.n_heavy = 0;
       __syncthreads();
       const uint32_t u = base + threadIdx.x;
@@ -521,20 +634,25 @@ int block_post_per_cu() {
 }
 
 int launch_block_subset(tgtp_ctx* ctx, const BlockArgs& A, const BlockSubset& S,
-                        uint32_t post_slots) {
+                        const uint32_t* round_tiles, uint32_t post_slots) {
   hipStream_t s = ctx->stream;
   const uint32_t nb = S.b1 - S.b0;
   const uint32_t col_blocks = (nb * kBlkCols + 3) / 4;
   const uint64_t zb = std::min<uint64_t>(16384, (S.floats + 255) / 256 + 1);
-  blk_zero_kernel<<<static_cast<uint32_t>(zb), 256, 0, s>>>(A.xa, S.floats);
+  blk_zero_kernel<<<static_cast<uint32_t>(zb), 256, 0, s>>>(A.xb, S.floats);
   blk_init_kernel<<<col_blocks, 256, 0, s>>>(A, S.b0, S.b1);
-  for (uint32_t r = 1; r <= S.rounds; ++r) {
+  blk_first_kernel<<<col_blocks, 256, 0, s>>>(A, S.b0, S.b1);
+  blk_stop_kernel<<<col_blocks, 256, 0, s>>>(A, S.b0, S.b1, A.xb);
+  const uint32_t tile_grid = (S.tiles + 7) / 8 * 8;
+  for (uint32_t r = 2; r <= S.rounds; ++r) {
     const float* cur = (r & 1u) ? A.xa : A.xb;
     float* nxt = (r & 1u) ? A.xb : A.xa;
-    blk_round_kernel<<<S.tiles, 256, 0, s>>>(A, S.b0, S.b1, cur, nxt);
+    const uint32_t tr = round_tiles[r];
+    if (tr == 0) continue;
+    blk_round_kernel<<<(tr + 7) / 8 * 8, 256, 0, s>>>(A, S.b0, S.b1, tr, cur, nxt);
     blk_stop_kernel<<<col_blocks, 256, 0, s>>>(A, S.b0, S.b1, nxt);
   }
-  blk_transpose_kernel<<<S.tiles, 256, 0, s>>>(A, S.b0, S.b1);
+  blk_transpose_kernel<<<tile_grid, 256, 0, s>>>(A, S.b0, S.b1, S.tiles);
   blk_zero_kernel<<<1, 256, 0, s>>>(reinterpret_cast<float*>(A.next_src), 1);
   blk_post_kernel<<<std::max(1u, std::min(post_slots, nb * kBlkCols)), kPB,
                     kPostLds * sizeof(float), s>>>(A, S.b0, S.b1);

@@ -7,7 +7,7 @@
 // and one diffusion round of all 64 sources is one sparse-times-dense
 // product with the graph's random-walk matrix.  Blocks are grouped into
 // subsets whose state fits the plan's buffers; a subset runs
-//   zero + init, R x (round, stop), transpose, post
+//   zero + init, sparse first round, (R - 1) x (round, stop), transpose, post
 // where R bounds the rounds any source of the subset can take (the largest
 // tau of its rows), the stop kernel applies the reference's cost rule per
 // source, and post does the top-gamma bound, chi, walk counts and 2-way walks
@@ -72,11 +72,14 @@ struct BlockSubset {
   uint32_t b0, b1;     // blocks [b0, b1)
   uint32_t tiles;      // round tiles (kBlkRows vertices of one block each)
   uint32_t rounds;     // rounds to launch (>= every source's stopping round)
+  uint32_t round_tiles0;  // index of this subset's per-round tile counts
   uint64_t floats;     // state floats per buffer
 };
 
 // Enqueues the whole per-source phase for one subset on ctx->stream.
+// round_tiles[r]: the tiles round r covers (the subset's blocks are ordered
+// by decreasing round bound, so they are a prefix).
 int launch_block_subset(tgtp_ctx* ctx, const BlockArgs& A, const BlockSubset& S,
-                        uint32_t post_slots);
+                        const uint32_t* round_tiles, uint32_t post_slots);
 // Resident workgroups per CU of the post kernel.
 int block_post_per_cu();

@@ -76,6 +76,7 @@ struct FastPlan {
   uint32_t n_blocks = 0;
   uint64_t n_sources = 0;
   std::vector<BlockSubset> subsets;
+  std::vector<uint32_t> round_tiles;  // per subset, rounds + 1 entries: tiles round r needs
   // scratch
   int32_t* d_taus = nullptr;
   double* d_preds = nullptr;
@@ -305,17 +306,42 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
     while (b < P->n_blocks) {
       const uint64_t need = static_cast<uint64_t>(blocks[b].n) * kBlkCols;
       if (b > b0 && fl + need > budget) break;
-      blocks[b].xoff = fl;
-      blocks[b].tile0 = tiles;
       fl += need;
-      tiles += (blocks[b].n + kBlkRows - 1) / kBlkRows;
-      rounds = std::max(rounds, bround[b]);
       ++b;
+    }
+    // blocks that may need more rounds first: round r then covers a prefix
+    // of the subset's tiles (the blocks whose round bound is >= r)
+    std::vector<uint32_t> ord(b - b0);
+    std::iota(ord.begin(), ord.end(), b0);
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](uint32_t x, uint32_t y) { return bround[x] > bround[y]; });
+    std::vector<BlockInfo> bs;
+    std::vector<uint32_t> br;
+    for (uint32_t x : ord) {
+      bs.push_back(blocks[x]);
+      br.push_back(bround[x]);
+    }
+    fl = 0;
+    for (uint32_t i = 0; i < bs.size(); ++i) {
+      blocks[b0 + i] = bs[i];
+      bround[b0 + i] = br[i];
+      blocks[b0 + i].xoff = fl;
+      blocks[b0 + i].tile0 = tiles;
+      fl += static_cast<uint64_t>(bs[i].n) * kBlkCols;
+      tiles += (bs[i].n + kBlkRows - 1) / kBlkRows;
+      rounds = std::max(rounds, br[i]);
     }
     S.b1 = b;
     S.tiles = tiles;
     S.rounds = rounds;
     S.floats = fl;
+    S.round_tiles0 = static_cast<uint32_t>(P->round_tiles.size());
+    for (uint32_t r = 0; r <= rounds; ++r) {
+      uint32_t tr = 0;
+      for (uint32_t i = 0; i < bs.size() && br[i] >= r; ++i)
+        tr = blocks[b0 + i].tile0 + (bs[i].n + kBlkRows - 1) / kBlkRows;
+      P->round_tiles.push_back(tr);
+    }
     max_floats = std::max(max_floats, fl);
     P->subsets.push_back(S);
     b0 = b;
@@ -402,7 +428,8 @@ int enqueue_run(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_
   A.light = P->light;
   A.small_bq = P->small_bq;
   for (const BlockSubset& S : P->subsets)
-    if ((rc = launch_block_subset(ctx, A, S, P->slots))) return rc;
+    if ((rc = launch_block_subset(ctx, A, S, P->round_tiles.data() + S.round_tiles0, P->slots)))
+      return rc;
   if (record_events) TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
   return launch_symmetrize_d(ctx, P->d_preds, src_of, P->m, P->d_out);
 }
@@ -438,11 +465,10 @@ int autotune(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_of)
     *knob = pick;
     return TGTP_OK;
   };
-  if ((rc = trial(&P->light, {16, 32, 64}))) return rc;
-  if ((rc = trial(&P->small_bq, {2, 8, 32}))) return rc;
+  if ((rc = trial(&P->light, {32, 64}))) return rc;
+  if ((rc = trial(&P->small_bq, {8, 32}))) return rc;
   const uint32_t full = P->max_slots;
-  if ((rc = trial(&P->slots, {full, std::max(1u, full / 2), std::max(1u, (3 * full) / 4)})))
-    return rc;
+  if ((rc = trial(&P->slots, {full, std::max(1u, (3 * full) / 4)}))) return rc;
   return TGTP_OK;
 }
 
