@@ -123,36 +123,8 @@ saba_walk_kernel(const uint2* __restrict__ rowd, const uint32_t* __restrict__ ad
     const uint2 r0 = rowd[active ? cur : 0u];
     uint32_t start = active ? r0.x : 0u, deg = active ? r0.y : 0u;
     bool live = active && deg != 0u;  // degree-0 sources are skipped
-    for (uint32_t hop = 0; hop < P.depth; ++hop) {
-      // a zero-degree vertex is reachable only in a non-symmetric CSR: the
-      // walker stops there instead of reading past its row
-      live = live && deg != 0u;
-      const uint64_t z = __ballot(live && !fb && sec == 0u);
-      fb = fb || (((z >> (grp * 8u)) & 0xFFull) != 0ull);
-      uint32_t vis = kNone;
-      if (live) {
-        uint32_t k;
-        if (!fb) {
-          k = __umulhi(sec ^ fmix32(cur), deg);
-          sec *= deg;
-        } else {
-          k = splitmix32(st) % deg;
-        }
-        if constexpr (kPacked) {
-          const uint64_t e = walk_adj[start + k];
-          cur = static_cast<uint32_t>(e & id_mask);
-          start = static_cast<uint32_t>((e >> P.id_bits) & start_mask);
-          deg = static_cast<uint32_t>(e >> deg_shift);
-        } else {
-          cur = adj[start + k];
-          if (hop + 1 < P.depth) {
-            const uint2 r = rowd[cur];
-            start = r.x;
-            deg = r.y;
-          }
-        }
-        vis = cur;
-      }
+    // Counts one visit (kNone: none).  Converged: every lane of the wave calls it.
+    auto count_visit = [&](uint32_t vis) {
       if (vis != kNone && vis < P.hot) atomicAdd(&hot_cnt[vis], 1u);
       if constexpr (!kBucket) {
         if (vis != kNone && vis >= P.hot)
@@ -166,7 +138,7 @@ saba_walk_kernel(const uint2* __restrict__ rowd, const uint32_t* __restrict__ ad
           pos = atomicAdd(&stage->cnt[wv][bk], 1u);
           stage->rec[wv][bk][pos] = vis;
         }
-        // converged: flush every bucket that reached kFlush in this hop
+        // flush every bucket that reached kFlush in this hop
         uint64_t need = __ballot(cold && pos == kFlush - 1u);
         while (need) {
           const int l = __builtin_ctzll(need);
@@ -190,6 +162,63 @@ saba_walk_kernel(const uint2* __restrict__ rowd, const uint32_t* __restrict__ ad
           }
           __builtin_amdgcn_wave_barrier();
         }
+      }
+    };
+    // Picks the next neighbour slot of a live lane (the reference's hash
+    // step, or the bouquet's SplitMix32 fallback once any lane's secret is 0).
+    auto step = [&](bool lv) -> uint32_t {
+      const uint64_t z = __ballot(lv && !fb && sec == 0u);
+      fb = fb || (((z >> (grp * 8u)) & 0xFFull) != 0ull);
+      uint32_t k = 0u;
+      if (lv) {
+        if (!fb) {
+          k = __umulhi(sec ^ fmix32(cur), deg);
+          sec *= deg;
+        } else {
+          k = splitmix32(st) % deg;
+        }
+      }
+      return k;
+    };
+    if constexpr (kPacked) {
+      // Software-pipelined: hop h+1's gather is issued before hop h's visit
+      // is counted, so the LDS atomics and staging of a visit overlap the
+      // next gather instead of preceding it.  A lane not walking reads entry
+      // 0 (m > 0 when the kernel runs).
+      const uint32_t k0 = step(live);  // every lane: the ballot inside must be converged
+      uint64_t e = walk_adj[live ? start + k0 : 0u];
+      for (uint32_t hop = 0; hop < P.depth; ++hop) {
+        uint32_t vis = kNone;
+        if (live) {
+          cur = static_cast<uint32_t>(e & id_mask);
+          start = static_cast<uint32_t>((e >> P.id_bits) & start_mask);
+          deg = static_cast<uint32_t>(e >> deg_shift);
+          vis = cur;
+        }
+        if (hop + 1 < P.depth) {
+          // a zero-degree vertex is reachable only in a non-symmetric CSR:
+          // the walker stops there instead of reading past its row
+          live = live && deg != 0u;
+          const uint32_t k = step(live);
+          e = walk_adj[live ? start + k : 0u];
+        }
+        count_visit(vis);
+      }
+    } else {
+      for (uint32_t hop = 0; hop < P.depth; ++hop) {
+        live = live && deg != 0u;
+        const uint32_t k = step(live);
+        uint32_t vis = kNone;
+        if (live) {
+          cur = adj[start + k];
+          if (hop + 1 < P.depth) {
+            const uint2 r = rowd[cur];
+            start = r.x;
+            deg = r.y;
+          }
+          vis = cur;
+        }
+        count_visit(vis);
       }
     }
     s_off += P.stride_src;
@@ -228,9 +257,13 @@ struct HistParams {
   uint32_t rbase[kMaxBuckets + 1];
 };
 
+constexpr int kCountBlock = 1024;  // 16 waves share one LDS histogram
+
 // blockIdx.x = (bucket * passes + pass) * slices + slice: counts the records
 // of waves [slice * waves / slices, ...) that fall in the pass's vertex range.
-__global__ void __launch_bounds__(256)
+// Records are read 4 at a time (regions start 128-byte aligned), two loads
+// in flight per thread, into one LDS histogram per workgroup.
+__global__ void __launch_bounds__(kCountBlock)
 bucket_count_kernel(const uint32_t* __restrict__ rec, const uint32_t* __restrict__ fill,
                     uint32_t* __restrict__ counters, HistParams Q) {
   __shared__ uint32_t hist[kHistBins];
@@ -242,20 +275,34 @@ bucket_count_kernel(const uint32_t* __restrict__ rec, const uint32_t* __restrict
   if (lo64 >= Q.n) return;  // uniform per block
   const uint32_t lo = static_cast<uint32_t>(lo64);
   const uint32_t span = static_cast<uint32_t>(min<uint64_t>(kHistBins, Q.n - lo64));
-  for (uint32_t i = threadIdx.x; i < kHistBins; i += 256) hist[i] = 0;
+  for (uint32_t i = threadIdx.x; i < kHistBins; i += kCountBlock) hist[i] = 0;
   __syncthreads();
+  auto add = [&](uint32_t v) {
+    const uint32_t d = v - lo;
+    if (d < span) atomicAdd(&hist[d], 1u);
+  };
   const uint32_t w0 = static_cast<uint32_t>(static_cast<uint64_t>(slice) * Q.waves / Q.slices);
   const uint32_t w1 = static_cast<uint32_t>(static_cast<uint64_t>(slice + 1) * Q.waves / Q.slices);
   for (uint32_t w = w0; w < w1; ++w) {
     const uint32_t cnt = fill[static_cast<uint64_t>(w) * Q.n_buckets + bk];
     const uint32_t* r = rec + static_cast<uint64_t>(w) * Q.rtot + Q.rbase[bk];
-    for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
-      const uint32_t d = r[i] - lo;
-      if (d < span) atomicAdd(&hist[d], 1u);
+    const uint4* r4 = reinterpret_cast<const uint4*>(r);
+    const uint32_t n4 = cnt / 4;
+    uint32_t i = threadIdx.x;
+    for (; i + kCountBlock < n4; i += 2 * kCountBlock) {
+      const uint4 a = r4[i];
+      const uint4 c = r4[i + kCountBlock];
+      add(a.x); add(a.y); add(a.z); add(a.w);
+      add(c.x); add(c.y); add(c.z); add(c.w);
     }
+    if (i < n4) {
+      const uint4 a = r4[i];
+      add(a.x); add(a.y); add(a.z); add(a.w);
+    }
+    if (threadIdx.x < cnt - 4 * n4) add(r[4 * n4 + threadIdx.x]);
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < span; i += 256) {
+  for (uint32_t i = threadIdx.x; i < span; i += kCountBlock) {
     const uint32_t c = hist[i];
     if (c) __hip_atomic_fetch_add(&counters[lo + i], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
@@ -481,8 +528,9 @@ int launch_saba_walks(tgtp_ctx* ctx, const tgtp_dgraph* cdg, uint32_t src_begin,
         Q.waves = static_cast<uint32_t>(waves);
         Q.rtot = P.rbase[nb];
         for (uint32_t k = 0; k <= nb; ++k) Q.rbase[k] = P.rbase[k];
+        // about two workgroups per CU (one fits at a time: 128 KiB of LDS)
         Q.slices = static_cast<uint32_t>(std::max<uint64_t>(
-            1, std::min<uint64_t>(waves, (4ull * ctx->num_cus + nb * passes - 1) / (nb * passes))));
+            1, std::min<uint64_t>(waves, (2ull * ctx->num_cus + nb * passes - 1) / (nb * passes))));
       } else {
         // fall back to direct atomics with the plain kernel
         P.n_buckets = 0;
@@ -520,7 +568,8 @@ int launch_saba_walks(tgtp_ctx* ctx, const tgtp_dgraph* cdg, uint32_t src_begin,
       TGTP_HIP_CHECK(ctx, hipLaunchKernel(fn, dim3(static_cast<uint32_t>(blocks)),
                                           dim3(kWalkBlock), args, lds_launch, s));
       if (bucket) {
-        bucket_count_kernel<<<nb * passes * Q.slices, 256, 0, s>>>(rec, fill, d_counters, Q);
+        bucket_count_kernel<<<nb * passes * Q.slices, kCountBlock, 0, s>>>(rec, fill, d_counters,
+                                                                           Q);
         TGTP_HIP_CHECK(ctx, hipGetLastError());
       }
     }
