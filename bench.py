@@ -86,10 +86,13 @@ def bench_saba(args):
                 f"n_walks={n_walks}")
     log(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step, walk kernel "
         f"{avg_walk_s * 1e3:.2f} ms, {value:.3e} transitions/s, checksum={cs}")
+    # one walk pass = source chunks x (saba_walk_kernel, bucket_count_kernel);
+    # the unit of "launch" here is the pass, timed over all its kernels
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_for("saba_walk_kernel", "cl1m_d10_w16384"),
-            "kernel": "saba_walk_kernel", "kernel_ms": round(avg_walk_s * 1e3, 3),
+            "traffic": traffic_for("saba_walk_pass", "cl1m_d10_w16384"),
+            "kernel": "saba_walk_kernel + bucket_count_kernel (one pass)",
+            "kernel_ms": round(avg_walk_s * 1e3, 3),
             "bytes_per_unit": BYTES_PER_TRANSITION, "units_per_launch": trans}
     cfg = {"workload": workload, "graph": WL.describe(WL.CL1M), "n": g.n, "m": g.m,
            "depth": depth, "n_walks": n_walks, "visit_count": vc, "checksum": cs}
@@ -367,6 +370,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tgtp", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-saba", action="store_true", help="tgtp legs only (profiling)")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-sample-edges", type=int, default=32)
     ap.add_argument("--batch-cpu-graphs", type=int, default=16)
@@ -377,6 +381,14 @@ def main() -> int:
               "design (source sharding + one counter all-reduce) is described in DESIGN.md "
               "and is not built", file=sys.stderr)
         return 2
+    if args.no_saba:  # profiling runs of the tgtp legs only; no headline value
+        line = {"metric": METRIC, "value": None, "unit": "transitions/s", "n_gpus": 1}
+        if not args.no_tgtp:
+            line["tgtp"] = bench_tgtp(args)
+        if not args.no_batch:
+            line["tgtp_batch"] = bench_batch(args)
+        print(json.dumps(line), flush=True)
+        return 0
     value, ms_per_step, roof, cfg, g = bench_saba(args)
     line = {"metric": METRIC, "value": value, "unit": "transitions/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,

@@ -21,11 +21,18 @@
 // the field widths fit, the walk reads a packed layout built once per graph:
 // walk_adj[p] = {neighbour id | neighbour row start | neighbour degree} in 64
 // bits, so one random 8-byte load per hop yields everything the next hop
-// needs (the plain layout needs the neighbour id, then its row descriptor).
-// Counters of vertices [0, hot) are staged in LDS per workgroup and flushed
-// once; visits to other vertices are device-scope atomics, which this part
-// performs memory-side (one 32-byte sector each), parked in registers and
-// issued once per chunk of hops so that they stay off the gather chain.
+// needs (the plain layout needs the neighbour id, then its row descriptor);
+// the hop loop is software-pipelined, hop h+1's gather in flight while hop
+// h's visit is counted.
+//
+// Counting: counters of vertices [0, hot) live in LDS per workgroup and are
+// flushed once.  A visit to another ("cold") vertex would be a device-scope
+// atomic, which this part performs memory-side (one 32-byte sector each);
+// instead it is staged in LDS per vertex-range bucket and written out 32 at a
+// time as a 128-byte record run into the wave's own region of HBM, and
+// bucket_count_kernel then counts each bucket's records in an LDS histogram.
+// Sources run in strided chunks so that a chunk's records fit the regions; a
+// region that fills up falls back to the atomic, so every count is exact.
 #include "common.hpp"
 
 #include <algorithm>
@@ -439,11 +446,26 @@ int launch_saba_walks(tgtp_ctx* ctx, const tgtp_dgraph* cdg, uint32_t src_begin,
     P.id_bits = packed ? dg->id_bits : 0;
     P.start_bits = packed ? dg->start_bits : 0;
     // LDS counters of the lowest vertex ids (the hubs of the Chung-Lu
-    // generator; correct for any graph, only the hit rate changes).
-    const uint32_t hot_max = static_cast<uint32_t>(env_int("TGTP_WALK_HOT", 32768));
+    // generator; correct for any graph, only the hit rate changes).  With
+    // bucketed cold counting the default is the largest table that leaves
+    // two workgroups per CU (more walks in flight beat a higher LDS hit
+    // rate: 14,208 counters measured 8% faster than 32,768 at n = 2^20);
+    // with direct atomics a cold visit is expensive and 32,768 wins.
+    const bool want_bucket = env_int("TGTP_WALK_BUCKETS", 1) != 0;
+    uint32_t hot_dflt = 32768;
+    if (want_bucket) {
+      int lds_cu = 0;
+      if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                ctx->device) != hipSuccess || lds_cu <= 0)
+        lds_cu = 65536;
+      const int64_t room = (static_cast<int64_t>(lds_cu) / 2 -
+                            static_cast<int64_t>(sizeof(StageLds))) / 4;
+      hot_dflt = static_cast<uint32_t>(std::max<int64_t>(0, room) / 64 * 64);
+    }
+    const uint32_t hot_max = static_cast<uint32_t>(env_int("TGTP_WALK_HOT", static_cast<int>(hot_dflt)));
     P.hot = static_cast<uint32_t>(std::min<uint64_t>(n, hot_max));
     // cold vertices in at most kMaxBuckets ranges of 2^bshift >= kHistBins
-    bool bucket = env_int("TGTP_WALK_BUCKETS", 1) != 0 && n > P.hot;
+    bool bucket = want_bucket && n > P.hot;
     uint32_t nb = 0, passes = 1;
     std::vector<double> mass;
     if (bucket) {
