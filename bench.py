@@ -335,6 +335,50 @@ def bench_batch(args):
     return out
 
 
+def bench_precision(args):
+    """Precision form of tgtp: the 512 x 512 grid with 5% of edges rewired
+    (n = 262,144), omega = 128, gamma = 10, eps 0.01 and 0.005, FAST_FP32
+    and FAST_BF16 (projection in FP32 / BF16 with FP32 accumulation, FP64
+    score sums); max / mean relative error against FP64 exact resistances of
+    sampled edges (block PCG).  No CPU leg: the reference algorithm needs
+    ~2.4e13 pushed edges here (hours on the host)."""
+    import numpy as np
+    import torch
+
+    from tgtp_amd import MODE_FAST_BF16, MODE_FAST_FP32, Context, eigenprep as E, exact_sc
+    from tgtp_amd import workloads as WL
+
+    side = args.precision_side
+    g = WL.G.grid_rewired(side, side, 0.05, seed=1)
+    t0 = time.time()
+    lam, X = E.eigens_f32(*E.compute_eigens_many([g], 128, device="cuda:0")[0])
+    t_eig = time.time() - t0
+    rng = np.random.default_rng(3)
+    pos = np.sort(rng.choice(g.m, size=min(args.precision_sample_edges, g.m), replace=False))
+    R = exact_sc.sampled_scores(g, pos, "cuda:0")
+    torch.cuda.empty_cache()
+    log(f"[bench] precision grid {side}x{side}: n={g.n} m={g.m}, eigenpairs {t_eig:.1f}s")
+    out = {"workload": f"grid {side}x{side}, 5% rewired, seed 1", "n": g.n, "m": g.m,
+           "omega": 128, "gamma": 10, "eigen_prep_s": round(t_eig, 1)}
+    with Context(0) as ctx:
+        dg = ctx.upload(g)
+        for eps in (0.01, 0.005):
+            leg = {}
+            for mode, name in ((MODE_FAST_FP32, "fast_fp32"), (MODE_FAST_BF16, "fast_bf16")):
+                t1 = time.perf_counter()
+                s, st = ctx.score(dg, lam, X, eps, 10, mode=mode)
+                dt = time.perf_counter() - t1
+                leg[name] = {"edges_per_s": g.m / dt, "seconds": dt,
+                             "taus_ms": st["taus_ms"], "diffuse_ms": st["diffuse_ms"],
+                             "vs_exact_fp64_sampled": error_stats(s, R, eps, pos)}
+                e = leg[name]["vs_exact_fp64_sampled"]
+                log(f"[bench] precision eps={eps} {name}: {dt:.1f}s, max rel {e['max_rel']:.4f} "
+                    f"mean rel {e['mean_rel']:.5f} max abs {e['max_abs']:.4f}")
+            out[f"eps_{eps}"] = leg
+        dg.free()
+    return out
+
+
 def cpu_saba_baseline(g, depth, n_walks, sample_sources: int):
     """The oracle's OpenMP walk (schedule(dynamic) over sources, per-thread
     counters) on evenly spread source slices of the same workload."""
@@ -371,6 +415,10 @@ def main() -> int:
     ap.add_argument("--no-tgtp", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-saba", action="store_true", help="tgtp legs only (profiling)")
+    ap.add_argument("--precision", action="store_true",
+                    help="add the 512x512 rewired-grid precision leg (minutes)")
+    ap.add_argument("--precision-side", type=int, default=512)
+    ap.add_argument("--precision-sample-edges", type=int, default=256)
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-sample-edges", type=int, default=32)
     ap.add_argument("--batch-cpu-graphs", type=int, default=16)
@@ -387,6 +435,8 @@ def main() -> int:
             line["tgtp"] = bench_tgtp(args)
         if not args.no_batch:
             line["tgtp_batch"] = bench_batch(args)
+        if args.precision:
+            line["tgtp_precision"] = bench_precision(args)
         print(json.dumps(line), flush=True)
         return 0
     value, ms_per_step, roof, cfg, g = bench_saba(args)
@@ -405,6 +455,8 @@ def main() -> int:
         line["tgtp"] = bench_tgtp(args)
     if not args.no_batch:
         line["tgtp_batch"] = bench_batch(args)
+    if args.precision:
+        line["tgtp_precision"] = bench_precision(args)
     print(json.dumps(line), flush=True)
     return 0
 

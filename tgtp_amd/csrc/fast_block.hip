@@ -322,6 +322,7 @@ struct PostShared {
   uint32_t k;
   uint32_t cand[kCandCap];
   uint32_t n_cand;
+  double wacc[kPB / 8];   // per 8-lane group walk sums
 };
 
 // One bouquet of the 2-way walk for lane b of an 8-lane group; returns the
@@ -567,23 +568,49 @@ __device__ void post_source(const BlockArgs& A, const BlockInfo& B, uint32_t col
     }
   }
   __syncthreads();
-  for (uint32_t p = r0; p < r1; ++p) {
-    const int rest = A.taus[p] - ell;
-    const uint64_t nr = rest > 0 ? A.nr_pos[p] : 0;
-    if (nr / 8 <= small_bq) continue;
-    const uint32_t v = A.adj[p];
-    const uint32_t step = static_cast<uint32_t>(0xFFFFFFFFull / nr);
-    const uint64_t nbq = nr / 8;
-    double acc = 0.0;
-    for (uint64_t jb = g; jb < nbq; jb += kPB / 8)
-      acc += group_sum8(walk2_bouquet(A, pv, off, src, v, static_cast<uint32_t>(jb), b, step,
-                                      rest, giw),
-                        gbase);
-    double mine = (b == 0) ? acc : 0.0;  // one contribution per group, groups in order
-    mine = block_reduce<kPW>(mine, sh.rd, OpSum{});
-    if (threadIdx.x == 0) {
-      A.preds_d[p] += mine / (dsd * static_cast<double>(nr));
-      if (A.tr_x) A.tr_x[p] = static_cast<float>(mine);
+  // Larger positions: four at a time, eight groups each.  Position slot i
+  // (groups 8i .. 8i+7) takes bouquets j = g % 8 + 8t of its position; its
+  // sum is the eight group sums in group order.  The list of larger
+  // positions is built in row order, 256 positions at a time.
+  for (uint32_t base = r0; base < r1; base += kPB) {
+    const uint32_t p0 = base + threadIdx.x;
+    bool big = false;
+    if (p0 < r1) {
+      const int rest = A.taus[p0] - ell;
+      const uint64_t nr = rest > 0 ? A.nr_pos[p0] : 0;
+      big = nr / 8 > small_bq;
+    }
+    uint32_t nbig = 0;
+    const uint32_t at = block_exclusive_scan<kPW>(big ? 1u : 0u, sh.ru, &nbig);
+    if (big) sh.heavy[at] = p0;
+    __syncthreads();
+    for (uint32_t q = 0; q < nbig; q += 4) {
+      const uint32_t slot = g >> 3, gi = g & 7u;
+      double acc = 0.0;
+      if (q + slot < nbig) {
+        const uint32_t p = sh.heavy[q + slot];
+        const int rest = A.taus[p] - ell;
+        const uint64_t nr = A.nr_pos[p];
+        const uint32_t v = A.adj[p];
+        const uint32_t step = static_cast<uint32_t>(0xFFFFFFFFull / nr);
+        const uint64_t nbq = nr / 8;
+        for (uint64_t jb = gi; jb < nbq; jb += 8)
+          acc += group_sum8(walk2_bouquet(A, pv, off, src, v, static_cast<uint32_t>(jb), b, step,
+                                          rest, giw),
+                            gbase);
+      }
+      if (b == 0) sh.wacc[g] = acc;
+      __syncthreads();
+      if (threadIdx.x < 4 && q + threadIdx.x < nbig) {
+        const uint32_t pp = sh.heavy[q + threadIdx.x];
+        double sum = 0.0;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) sum += sh.wacc[8 * threadIdx.x + k];
+        const uint64_t nrr = A.nr_pos[pp];
+        A.preds_d[pp] += sum / (dsd * static_cast<double>(nrr));
+        if (A.tr_x) A.tr_x[pp] = static_cast<float>(sum);
+      }
+      __syncthreads();
     }
   }
   if (threadIdx.x == 0) {
