@@ -348,10 +348,14 @@ def bench_precision(args):
     from tgtp_amd import MODE_FAST_BF16, MODE_FAST_FP32, Context, eigenprep as E, exact_sc
     from tgtp_amd import workloads as WL
 
+    os.environ["TGTP_PROGRESS"] = "1"  # calls of minutes: the library reports each subset
     side = args.precision_side
     g = WL.G.grid_rewired(side, side, 0.05, seed=1)
     t0 = time.time()
-    lam, X = E.eigens_f32(*E.compute_eigens_many([g], 128, device="cuda:0")[0])
+    lam, X = E.eigens_f32(*E.compute_eigens_many(
+        [g], 128, device="cuda:0",
+        on_cycle=lambda c, r: log(f"[bench] precision eigen cycle {c}: residual {r:.2e} "
+                                  f"({time.time() - t0:.0f}s)") if c % 5 == 0 else None)[0])
     t_eig = time.time() - t0
     rng = np.random.default_rng(3)
     pos = np.sort(rng.choice(g.m, size=min(args.precision_sample_edges, g.m), replace=False))

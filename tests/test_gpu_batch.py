@@ -106,3 +106,21 @@ def test_batch_rejects_mixed_inputs(gpu_ctx, small_batch):
         gpu_ctx.score_batch(graphs[:2], [eig[0], (lam[:16], X[:, :16])], 0.05, 10)
     with pytest.raises(TgtpError, match="epsilon"):
         gpu_ctx.score_batch(graphs[:2], eig[:2], 1.5, 10)
+
+
+def test_batch_on_two_streams_equals_per_graph(gpu_ctx, small_batch, monkeypatch):
+    """Small state buffers split the batch into many subsets, which a batch
+    runs alternately on two streams (captured as a fork-join graph); the
+    scores still equal the per-graph ones bit for bit."""
+    graphs, eig = small_batch
+    monkeypatch.setenv("TGTP_FAST_STATE_MIB", "8")
+    ref = []
+    for g, (lam, X) in zip(graphs, eig):
+        s, _ = gpu_ctx.score(g, lam, X, 0.05, 10, mode=MODE_FAST_FP32)
+        ref.append(s)
+    b = gpu_ctx.batch(graphs, eig, 0.05, 10, mode=MODE_FAST_FP32, graph=True)
+    for rep in range(2):
+        outs, _ = b.run()
+        for k, (a, r) in enumerate(zip(outs, ref)):
+            assert np.array_equal(a.view(np.uint32), r.view(np.uint32)), f"replay {rep} graph {k}"
+    b.close()

@@ -121,6 +121,20 @@ def test_fast_on_rewired_grid_tracks_faithful(gpu_ctx):
         assert err.mean() <= ferr.mean() + 0.025
 
 
+def test_fast_tile_orders_agree(gpu_ctx, monkeypatch):
+    """Both orders of the diffusion tiles (per-XCD runs of blocks, and
+    groups of 8 tiles of one block across the XCDs, used for blocks above
+    65,536 vertices) compute the same scores."""
+    g = G.chung_lu(5000, 2.3, 10.0, seed=31)
+    lam, X = E.eigens_f32(*E.compute_eigens(g, 32))
+    out = []
+    for mode in ("0", "1"):
+        monkeypatch.setenv("TGTP_FAST_TILE_MODE", mode)
+        s, _ = gpu_ctx.score(g, lam, X, 0.02, 10, mode=MODE_FAST_FP32)
+        out.append(s)
+    assert np.array_equal(out[0].view(np.uint32), out[1].view(np.uint32))
+
+
 def test_fast_subsets_do_not_change_scores(gpu_ctx, monkeypatch):
     """Sources advance in blocks of 64 whose states share subset buffers; the
     scores do not depend on how blocks are grouped into subsets."""

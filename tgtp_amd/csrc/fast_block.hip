@@ -38,16 +38,21 @@ constexpr uint32_t kTopK = 16;         // single-pass top-gamma for gamma <= 16
 constexpr double kMaxWalksFast = 68719476736.0;
 constexpr uint32_t kInfBits = 0x7f800000u;
 
-// Tile of this workgroup.  Workgroups are dealt to the 8 XCDs round-robin
-// (blockIdx.x % 8); each XCD gets a contiguous run of tiles, so the blocks
-// whose state rows its L2 caches are few.  The grid is 8 * ceil(tiles / 8);
-// a workgroup past the last tile returns.
-__device__ __forceinline__ uint32_t xcd_tile() {
-  const uint32_t per = gridDim.x >> 3;
-  return (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
-}
+// Tile of this workgroup (returns false for a workgroup past the last tile;
+// workgroups are dealt to the 8 XCDs round-robin, blockIdx.x % 8).
+//  * tile_mode 0 (blocks of up to 65,536 vertices): XCD x takes the x-th
+//    contiguous eighth of the subset's tiles, so each XCD works through
+//    whole blocks of its own and its L2 holds their hot rows;
+//  * tile_mode 1 (larger blocks, whose n x 64 states would not fit the
+//    Infinity Cache eight at a time): tile groups -- 8 consecutive tiles of
+//    one block, one per XCD -- run in block order, so all XCDs share one
+//    block's state.
+// tile0 of a block counts tiles (mode 0) or tile groups (mode 1) before it.
+__device__ __forceinline__ bool tile_of(const BlockInfo* blocks, uint32_t b0, uint32_t b1,
+                                        uint32_t n_units, uint32_t mode, uint32_t* bo,
+                                        uint32_t* row0);
 
-// Last block in [b0, b1) whose first round tile is <= t.
+// Last block in [b0, b1) whose first tile unit is <= t.// Last block in [b0, b1) whose first tile unit is <= t.
 __device__ __forceinline__ uint32_t find_block(const BlockInfo* B, uint32_t b0, uint32_t b1,
                                                uint32_t t) {
   uint32_t lo = b0, hi = b1;
@@ -59,6 +64,27 @@ __device__ __forceinline__ uint32_t find_block(const BlockInfo* B, uint32_t b0, 
       hi = mid;
   }
   return lo;
+}
+
+__device__ __forceinline__ bool tile_of(const BlockInfo* blocks, uint32_t b0, uint32_t b1,
+                                        uint32_t n_units, uint32_t mode, uint32_t* bo,
+                                        uint32_t* row0) {
+  uint32_t unit, t;
+  if (mode == 0) {
+    const uint32_t per = gridDim.x >> 3;
+    unit = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+    if (unit >= n_units) return false;
+  } else {
+    unit = blockIdx.x >> 3;
+    if (unit >= n_units) return false;
+  }
+  const uint32_t b = find_block(blocks, b0, b1, unit);
+  const uint32_t tiles = (blocks[b].n + kBlkRows - 1) / kBlkRows;
+  t = mode == 0 ? unit - blocks[b].tile0 : 8u * (unit - blocks[b].tile0) + (blockIdx.x & 7u);
+  if (t >= tiles) return false;
+  *bo = b;
+  *row0 = t * kBlkRows;
+  return true;
 }
 
 __global__ void blk_zero_kernel(float* __restrict__ x, uint64_t n) {
@@ -144,15 +170,14 @@ __global__ void __launch_bounds__(256) blk_first_kernel(BlockArgs A, uint32_t b0
 // overlaps a finishing workgroup with the next one, a loop with a barrier
 // per tile does not).
 __global__ void __launch_bounds__(256)
-blk_round_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_tiles,
+blk_round_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_units,
                  const float* __restrict__ cur, float* __restrict__ nxt) {
   __shared__ float s_mx[4][kBlkCols];
   __shared__ float s_mn[4][kBlkCols];
   __shared__ uint32_t s_q[4][kBlkCols];
   __shared__ unsigned long long s_ds[4][kBlkCols];
-  const uint32_t t = xcd_tile();
-  if (t >= n_tiles) return;  // uniform per workgroup
-  const uint32_t b = find_block(A.blocks, b0, b1, t);
+  uint32_t b, row0;
+  if (!tile_of(A.blocks, b0, b1, n_units, A.tile_mode, &b, &row0)) return;  // uniform per wg
   const BlockInfo B = A.blocks[b];
   const uint32_t j = lane_id(), wv = wave_id();
   const uint32_t col = b * kBlkCols + j;
@@ -163,7 +188,6 @@ blk_round_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_tiles,
   float mx = 0.0f, mn = INFINITY;
   uint32_t q = 0;
   unsigned long long ds = 0;
-  const uint32_t row0 = (t - B.tile0) * kBlkRows;
   const uint32_t rend = min(row0 + kBlkRows, B.n);
   const float* Xc = cur + B.xoff;
   float* Xn = nxt + B.xoff;
@@ -286,13 +310,11 @@ blk_stop_kernel(BlockArgs A, uint32_t b0, uint32_t b1, const float* __restrict__
 // vector per source in the other buffer.  Also flags a source still active
 // after the last launched round (the round bound was wrong).
 __global__ void __launch_bounds__(256)
-blk_transpose_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_tiles) {
+blk_transpose_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_units) {
   __shared__ float tile[kBlkRows][kBlkCols + 1];
-  const uint32_t t = xcd_tile();
-  if (t >= n_tiles) return;  // uniform per workgroup
-  const uint32_t b = find_block(A.blocks, b0, b1, t);
+  uint32_t b, row0;
+  if (!tile_of(A.blocks, b0, b1, n_units, A.tile_mode, &b, &row0)) return;  // uniform per wg
   const BlockInfo B = A.blocks[b];
-  const uint32_t row0 = (t - B.tile0) * kBlkRows;
   const uint32_t L = A.last[b];
   const float* F = ((L & 1u) ? A.xb : A.xa) + B.xoff;
   float* O = ((L & 1u) ? A.xa : A.xb) + B.xoff;
@@ -660,9 +682,8 @@ int block_post_per_cu() {
   return std::max(1, per_cu);
 }
 
-int launch_block_subset(tgtp_ctx* ctx, const BlockArgs& A, const BlockSubset& S,
+int launch_block_subset(tgtp_ctx* ctx, hipStream_t s, const BlockArgs& A, const BlockSubset& S,
                         const uint32_t* round_tiles, uint32_t post_slots) {
-  hipStream_t s = ctx->stream;
   const uint32_t nb = S.b1 - S.b0;
   const uint32_t col_blocks = (nb * kBlkCols + 3) / 4;
   const uint64_t zb = std::min<uint64_t>(16384, (S.floats + 255) / 256 + 1);
@@ -670,13 +691,16 @@ int launch_block_subset(tgtp_ctx* ctx, const BlockArgs& A, const BlockSubset& S,
   blk_init_kernel<<<col_blocks, 256, 0, s>>>(A, S.b0, S.b1);
   blk_first_kernel<<<col_blocks, 256, 0, s>>>(A, S.b0, S.b1);
   blk_stop_kernel<<<col_blocks, 256, 0, s>>>(A, S.b0, S.b1, A.xb);
-  const uint32_t tile_grid = (S.tiles + 7) / 8 * 8;
+  // tile units: tiles (mode 0, grid rounded up to a multiple of 8) or
+  // groups of 8 tiles (mode 1)
+  auto grid_for = [&](uint32_t units) { return A.tile_mode == 0 ? (units + 7) / 8 * 8 : 8 * units; };
+  const uint32_t tile_grid = grid_for(S.tiles);
   for (uint32_t r = 2; r <= S.rounds; ++r) {
     const float* cur = (r & 1u) ? A.xa : A.xb;
     float* nxt = (r & 1u) ? A.xb : A.xa;
     const uint32_t tr = round_tiles[r];
     if (tr == 0) continue;
-    blk_round_kernel<<<(tr + 7) / 8 * 8, 256, 0, s>>>(A, S.b0, S.b1, tr, cur, nxt);
+    blk_round_kernel<<<grid_for(tr), 256, 0, s>>>(A, S.b0, S.b1, tr, cur, nxt);
     blk_stop_kernel<<<col_blocks, 256, 0, s>>>(A, S.b0, S.b1, nxt);
   }
   blk_transpose_kernel<<<tile_grid, 256, 0, s>>>(A, S.b0, S.b1, S.tiles);

@@ -28,7 +28,8 @@ struct BlockInfo {
   uint32_t src0;   // first source of the block (graph-local id)
   uint32_t cnt;    // sources in the block (<= 64)
   uint32_t gi;     // graph index
-  uint32_t tile0;  // round tiles of the subset before this block
+  uint32_t tile0;  // tile units of the subset before this block: tiles
+                   // (tile_mode 0) or groups of 8 consecutive tiles (mode 1)
   uint64_t xoff;   // float offset of the block's n x 64 state in the subset buffers
 };
 
@@ -58,6 +59,7 @@ struct BlockArgs {
   unsigned long long* st_dsum;
   uint32_t* last;      // per block: last round any of its sources took
   uint32_t* next_src;  // post kernel work counter
+  uint32_t tile_mode;  // 0: per-XCD runs of tiles; 1: per-block groups of 8 tiles
   uint32_t light;      // degree up to which one thread scans a vertex's row
   uint32_t small_bq;   // walk positions with at most this many bouquets: one 8-lane group
   int32_t* tr_ell;
@@ -70,16 +72,16 @@ struct BlockArgs {
 
 struct BlockSubset {
   uint32_t b0, b1;     // blocks [b0, b1)
-  uint32_t tiles;      // round tiles (kBlkRows vertices of one block each)
+  uint32_t tiles;      // tile units (see BlockInfo::tile0)
   uint32_t rounds;     // rounds to launch (>= every source's stopping round)
   uint32_t round_tiles0;  // index of this subset's per-round tile counts
   uint64_t floats;     // state floats per buffer
 };
 
-// Enqueues the whole per-source phase for one subset on ctx->stream.
-// round_tiles[r]: the tiles round r covers (the subset's blocks are ordered
-// by decreasing round bound, so they are a prefix).
-int launch_block_subset(tgtp_ctx* ctx, const BlockArgs& A, const BlockSubset& S,
-                        const uint32_t* round_tiles, uint32_t post_slots);
+// Enqueues the whole per-source phase for one subset on `stream`.
+// round_tiles[r]: the tile units round r covers (the subset's blocks are
+// ordered by decreasing round bound, so they are a prefix).
+int launch_block_subset(tgtp_ctx* ctx, hipStream_t stream, const BlockArgs& A,
+                        const BlockSubset& S, const uint32_t* round_tiles, uint32_t post_slots);
 // Resident workgroups per CU of the post kernel.
 int block_post_per_cu();

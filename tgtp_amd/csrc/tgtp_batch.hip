@@ -21,6 +21,7 @@
 // with HIP events on a full run; the best is kept for the captured graph.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -91,17 +92,32 @@ struct FastPlan {
   uint32_t* d_words = nullptr;
   float* d_out = nullptr;
   uint32_t max_slots = 0, slots = 0;  // resident workgroups of the post kernel
+  uint32_t tile_mode = 0;
   // tuned parameters
   uint32_t light = 32, small_bq = 8;
   // captured run
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // batches: odd subsets on a second stream with their own state buffers,
+  // so one subset's post kernel (latency-bound walks) overlaps the next
+  // subset's rounds (bandwidth-bound)
+  bool dual = false;
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  float* d_sa2 = nullptr;
+  float* d_sb2 = nullptr;
+  uint64_t state_floats = 0;
 };
 
 void free_plan(FastPlan* P) {
   if (!P) return;
   if (P->exec) (void)hipGraphExecDestroy(P->exec);
   if (P->graph) (void)hipGraphDestroy(P->graph);
+  if (P->side) (void)hipStreamDestroy(P->side);
+  if (P->fork) (void)hipEventDestroy(P->fork);
+  if (P->join) (void)hipEventDestroy(P->join);
+  if (P->d_sa2) (void)hipFree(P->d_sa2);
+  if (P->d_sb2) (void)hipFree(P->d_sb2);
   void* ptrs[] = {P->d_voff, P->d_eps, P->d_logmd, P->d_gamma, P->d_tgraphs, P->d_und,
                   P->d_tile_graph, P->d_wt, P->d_wtb, P->d_xb, P->d_blocks, P->d_taus,
                   P->d_preds, P->d_nr, P->d_sa, P->d_sb, P->d_active, P->d_ell, P->d_colf,
@@ -293,6 +309,18 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
   }
   P->n_blocks = static_cast<uint32_t>(blocks.size());
   P->n_sources = dg->n;
+  // tile order: per-XCD runs of whole blocks, or (a block's state larger
+  // than 16 MiB, eight of which would not fit the Infinity Cache) groups of
+  // 8 tiles of one block across the XCDs
+  uint32_t n_max = 0;
+  for (const BlockInfo& B : blocks) n_max = std::max(n_max, B.n);
+  P->tile_mode = static_cast<uint64_t>(n_max) * kBlkCols * 4 > (16ull << 20) ? 1u : 0u;
+  const int tm = env_int_batch("TGTP_FAST_TILE_MODE", -1);
+  if (tm == 0 || tm == 1) P->tile_mode = static_cast<uint32_t>(tm);
+  auto units_of = [&](uint32_t n) {
+    const uint32_t t = (n + kBlkRows - 1) / kBlkRows;
+    return P->tile_mode == 0 ? t : (t + 7) / 8;
+  };
   // subsets of blocks whose states fit one state buffer
   const uint64_t budget =
       static_cast<uint64_t>(std::max(1, env_int_batch("TGTP_FAST_STATE_MIB", 2048))) << 18;  // floats
@@ -328,7 +356,7 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
       blocks[b0 + i].xoff = fl;
       blocks[b0 + i].tile0 = tiles;
       fl += static_cast<uint64_t>(bs[i].n) * kBlkCols;
-      tiles += (bs[i].n + kBlkRows - 1) / kBlkRows;
+      tiles += units_of(bs[i].n);
       rounds = std::max(rounds, br[i]);
     }
     S.b1 = b;
@@ -339,7 +367,7 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
     for (uint32_t r = 0; r <= rounds; ++r) {
       uint32_t tr = 0;
       for (uint32_t i = 0; i < bs.size() && br[i] >= r; ++i)
-        tr = blocks[b0 + i].tile0 + (bs[i].n + kBlkRows - 1) / kBlkRows;
+        tr = blocks[b0 + i].tile0 + units_of(bs[i].n);
       P->round_tiles.push_back(tr);
     }
     max_floats = std::max(max_floats, fl);
@@ -348,7 +376,7 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
   }
   const uint64_t cols = static_cast<uint64_t>(P->n_blocks) * kBlkCols;
   if ((rc = dev_upload(ctx, &P->d_blocks, blocks, "source blocks")) ||
-      (rc = dev_alloc(ctx, &P->d_sa, max_floats, "diffusion state")) ||
+      (rc = dev_alloc(ctx, &P->d_sa, (P->state_floats = max_floats), "diffusion state")) ||
       (rc = dev_alloc(ctx, &P->d_sb, max_floats, "diffusion state")) ||
       (rc = dev_alloc(ctx, &P->d_active, cols, "column flags")) ||
       (rc = dev_alloc(ctx, &P->d_ell, cols, "column rounds")) ||
@@ -392,6 +420,7 @@ BlockArgs plan_args(const FastPlan* P, tgtp_dgraph* dg) {
   A.next_src = P->d_words;
   A.light = P->light;
   A.small_bq = P->small_bq;
+  A.tile_mode = P->tile_mode;
   return A;
 }
 
@@ -427,11 +456,59 @@ int enqueue_run(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_
   BlockArgs A = override_args ? *override_args : plan_args(P, dg);
   A.light = P->light;
   A.small_bq = P->small_bq;
-  for (const BlockSubset& S : P->subsets)
-    if ((rc = launch_block_subset(ctx, A, S, P->round_tiles.data() + S.round_tiles0, P->slots)))
+  const bool dual = P->dual && P->subsets.size() > 1;
+  BlockArgs A2 = A;
+  if (dual) {
+    A2.xa = P->d_sa2;
+    A2.xb = P->d_sb2;
+    A2.next_src = P->d_words + 4;
+    TGTP_HIP_CHECK(ctx, hipEventRecord(P->fork, ctx->stream));
+    TGTP_HIP_CHECK(ctx, hipStreamWaitEvent(P->side, P->fork, 0));
+  }
+  // TGTP_PROGRESS=1: an uncaptured run waits for each subset and reports it
+  // on stderr (runs of minutes on large graphs)
+  bool progress = env_int_batch("TGTP_PROGRESS", 0) != 0;
+  if (progress) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    progress = hipStreamIsCapturing(ctx->stream, &cs) == hipSuccess &&
+               cs == hipStreamCaptureStatusNone;
+  }
+  const double t_start = tgtp_now_ms();
+  for (size_t i = 0; i < P->subsets.size(); ++i) {
+    const BlockSubset& S = P->subsets[i];
+    const bool second = dual && (i & 1u);
+    hipStream_t st = second ? P->side : ctx->stream;
+    if ((rc = launch_block_subset(ctx, st, second ? A2 : A, S,
+                                  P->round_tiles.data() + S.round_tiles0, P->slots)))
       return rc;
+    if (progress) {
+      TGTP_HIP_CHECK(ctx, hipStreamSynchronize(st));
+      std::fprintf(stderr, "tgtp: source block subset %zu/%zu (%u rounds) done at %.1f s\n", i + 1,
+                   P->subsets.size(), S.rounds, (tgtp_now_ms() - t_start) / 1e3);
+      std::fflush(stderr);
+    }
+  }
+  if (dual) {
+    TGTP_HIP_CHECK(ctx, hipEventRecord(P->join, P->side));
+    TGTP_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, P->join, 0));
+  }
   if (record_events) TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
   return launch_symmetrize_d(ctx, P->d_preds, src_of, P->m, P->d_out);
+}
+
+// Second stream and state buffers for a batch plan with more than one
+// subset (TGTP_BATCH_STREAMS=1 keeps one stream).
+int enable_dual(tgtp_ctx* ctx, FastPlan* P) {
+  if (P->subsets.size() < 2 || env_int_batch("TGTP_BATCH_STREAMS", 2) < 2) return TGTP_OK;
+  int rc;
+  if ((rc = dev_alloc(ctx, &P->d_sa2, P->state_floats, "diffusion state")) ||
+      (rc = dev_alloc(ctx, &P->d_sb2, P->state_floats, "diffusion state")))
+    return rc;
+  TGTP_HIP_CHECK(ctx, hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking));
+  TGTP_HIP_CHECK(ctx, hipEventCreateWithFlags(&P->fork, hipEventDisableTiming));
+  TGTP_HIP_CHECK(ctx, hipEventCreateWithFlags(&P->join, hipEventDisableTiming));
+  P->dual = true;
+  return TGTP_OK;
 }
 
 int timed_run(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_of, float* ms) {
@@ -721,6 +798,7 @@ extern "C" int tgtp_batch_create(tgtp_ctx* ctx, int32_t count, const tgtp_graph*
   b->dg->eig_omega = omega;
   b->dg->eig_key = b;  // never matches a caller pointer
   if ((rc = build_plan(ctx, b->dg, specs, omega, mode, &b->plan))) return fail(rc);
+  if ((rc = enable_dual(ctx, b->plan))) return fail(rc);
   auto* prep = static_cast<TgtpPrep*>(b->dg->prep);
   if (flags & TGTP_BATCH_AUTOTUNE) {
     const double t0 = tgtp_now_ms();
