@@ -414,7 +414,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--depth", type=int, default=10)
     ap.add_argument("--walks", type=int, default=16384)
-    ap.add_argument("--cpu-sample-sources", type=int, default=8192)
+    ap.add_argument("--cpu-sample-sources", type=int, default=32768)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tgtp", action="store_true")
     ap.add_argument("--no-batch", action="store_true")

@@ -195,6 +195,7 @@ blk_round_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_units,
   // made uniform, they are scalar loads and each neighbour costs one
   // 256-byte vector load of its state row
   const uint32_t wu = __builtin_amdgcn_readfirstlane(wv);
+  // (Two rows per wave with interleaved gathers measured 8-20% slower.)
   for (uint32_t u = row0 + wu; u < rend; u += 4) {
     const uint64_t ro = static_cast<uint64_t>(u) * kBlkCols + j;
     if (act) {
