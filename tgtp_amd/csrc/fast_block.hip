@@ -52,7 +52,7 @@ __device__ __forceinline__ bool tile_of(const BlockInfo* blocks, uint32_t b0, ui
                                         uint32_t n_units, uint32_t mode, uint32_t* bo,
                                         uint32_t* row0);
 
-// Last block in [b0, b1) whose first tile unit is <= t.// Last block in [b0, b1) whose first tile unit is <= t.
+// Last block in [b0, b1) whose first tile unit is <= t.
 __device__ __forceinline__ uint32_t find_block(const BlockInfo* B, uint32_t b0, uint32_t b1,
                                                uint32_t t) {
   uint32_t lo = b0, hi = b1;
@@ -253,6 +253,166 @@ blk_round_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_units,
       atomicMin(&A.st_mn[col], __float_as_uint(mn));
       atomicAdd(&A.st_q[col], q);
       atomicAdd(&A.st_dsum[col], ds);
+    }
+  }
+}
+
+constexpr uint32_t kOutOfRange = 0xfffffff0u;  // buffer offset past any state
+constexpr int kRsrcWord3 = 0x00020000;         // gfx9 raw buffer: 32-bit data, no swizzle
+
+// kC neighbours k .. k + kC - 1 of the quarter's row added to this lane's
+// four sums: lane ql < kC of the quarter fetches id k + ql, the ids are read
+// across the quarter, then all kC 16-byte gathers are issued before the adds.
+// Slots past the row's degree (and lanes whose four sources have all
+// stopped) read out of range: no memory access, and the zeros they return
+// leave the non-negative sums unchanged.
+template <int kC>
+__device__ __forceinline__ void gather_step(__amdgpu_buffer_rsrc_t xrs, const uint32_t* nb,
+                                            uint32_t voff, uint32_t deg, bool any, uint32_t l,
+                                            uint32_t k, float4& sum) {
+  const uint32_t ql = l & 15u;
+  uint32_t my = 0;
+  if (ql < static_cast<uint32_t>(kC) && k + ql < deg) my = nb[k + ql] - voff;
+  uint32_t off[kC];
+#pragma unroll
+  for (int c = 0; c < kC; ++c) {
+    const uint32_t id = __shfl(my, static_cast<int>((l & 48u) | static_cast<uint32_t>(c)));
+    off[c] = any && k + c < deg ? id * (kBlkCols * 4u) + ql * 16u : kOutOfRange;
+  }
+  float4 v[kC];
+#pragma unroll
+  for (int c = 0; c < kC; ++c)
+    v[c] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(xrs, off[c], 0, 0));
+#pragma unroll
+  for (int c = 0; c < kC; ++c) {
+    sum.x += v[c].x;
+    sum.y += v[c].y;
+    sum.z += v[c].z;
+    sum.w += v[c].w;
+  }
+}
+
+// The same round with four rows per wave: each 16-lane quarter takes one
+// row and each lane four sources (16-byte loads), so one load instruction
+// gathers a neighbour row for four vertices at once and a wave keeps up to
+// 4 x kChunk gathers in flight on low-degree rows, where the one-row form
+// has only deg of them.  Each source's sum runs in adjacency order exactly
+// as in blk_round_kernel, so both forms give the same bits.
+template <int kChunk, int kWaves>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves)))
+blk_round4_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_units,
+                  const float* __restrict__ cur, float* __restrict__ nxt) {
+  constexpr int kQuarters = 16;  // 4 waves x 4 quarters
+  __shared__ float s_mx[kQuarters][kBlkCols];
+  __shared__ float s_mn[kQuarters][kBlkCols];
+  __shared__ uint32_t s_q[kQuarters][kBlkCols];
+  __shared__ unsigned long long s_ds[kQuarters][kBlkCols];
+  uint32_t b, row0;
+  if (!tile_of(A.blocks, b0, b1, n_units, A.tile_mode, &b, &row0)) return;  // uniform per wg
+  const BlockInfo B = A.blocks[b];
+  const uint32_t l = lane_id(), wv = wave_id();
+  const uint32_t qd = l >> 4, ql = l & 15u, c0 = ql * 4u;
+  const uchar4 a4 = *reinterpret_cast<const uchar4*>(A.active + b * kBlkCols + c0);
+  const bool act[4] = {a4.x != 0, a4.y != 0, a4.z != 0, a4.w != 0};
+  const bool any = act[0] || act[1] || act[2] || act[3];
+  const bool all = act[0] && act[1] && act[2] && act[3];
+  if (__ballot(any) == 0ull) return;  // same flags in every wave
+  float mx[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  float mn[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+  uint32_t q[4] = {0, 0, 0, 0};
+  unsigned long long ds[4] = {0, 0, 0, 0};
+  const uint32_t rend = min(row0 + kBlkRows, B.n);
+  const float* Xc = cur + B.xoff + c0;
+  float* Xn = nxt + B.xoff + c0;
+  const uint32_t voff = B.voff;
+  // the block's state as a buffer resource (n x 256 bytes; the host keeps
+  // this form to n < 2^24): a gather is a 16-byte buffer load at lane offset
+  // id * 256 + 16 * ql
+  const __amdgpu_buffer_rsrc_t xrs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float*>(cur + B.xoff), 0, static_cast<int>(B.n * (kBlkCols * 4u)), kRsrcWord3);
+  // quarter qd of wave wv takes rows row0 + 16 i + 4 wv + qd: a wave's four
+  // rows are consecutive (similar degrees) and its stores one 1 KB run
+  for (uint32_t base = row0 + 4u * __builtin_amdgcn_readfirstlane(wv); base < rend; base += 16u) {
+    const uint32_t u = base + qd;
+    const bool rv = u < rend;
+    uint2 r = make_uint2(0u, 0u);
+    if (rv) r = A.rowd[voff + u];
+    // the row's degree is the same in all 16 lanes of the quarter, whatever
+    // their sources' flags: every lane fetches its share of the ids
+    const uint32_t deg = r.y;
+    const uint32_t dmax = max(max(__builtin_amdgcn_readlane(deg, 0), __builtin_amdgcn_readlane(deg, 16)),
+                              max(__builtin_amdgcn_readlane(deg, 32), __builtin_amdgcn_readlane(deg, 48)));
+    const uint32_t* nb = A.adj + r.x;
+    float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // steps of kChunk gathers while a row of the wave has more than half a
+    // chunk left, then of 8 and 4 (fewer idle slots on short rows)
+    for (uint32_t k = 0; k < dmax;) {
+      const uint32_t rem = dmax - k;
+      if (kChunk >= 16 && rem > 8) {
+        gather_step<16>(xrs, nb, voff, deg, any, l, k, sum);
+        k += 16;
+      } else if (rem > 4) {
+        gather_step<8>(xrs, nb, voff, deg, any, l, k, sum);
+        k += 8;
+      } else {
+        gather_step<4>(xrs, nb, voff, deg, any, l, k, sum);
+        k += 4;
+      }
+    }
+    if (rv) {
+      const uint64_t ro = static_cast<uint64_t>(u) * kBlkCols;
+      float4 val;
+      if (any) {
+        const float d = A.degf[voff + u];
+        val = make_float4(sum.x / d, sum.y / d, sum.z / d, sum.w / d);
+      }
+      if (!all) {
+        const float4 old = *reinterpret_cast<const float4*>(Xc + ro);
+        if (!act[0]) val.x = old.x;
+        if (!act[1]) val.y = old.y;
+        if (!act[2]) val.z = old.z;
+        if (!act[3]) val.w = old.w;
+      }
+      *reinterpret_cast<float4*>(Xn + ro) = val;
+      const float vv[4] = {val.x, val.y, val.z, val.w};
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (act[i] && vv[i] > 0.0f) {
+          mx[i] = fmaxf(mx[i], vv[i]);
+          mn[i] = fminf(mn[i], vv[i]);
+          ++q[i];
+          ds[i] += r.y;
+        }
+      }
+    }
+  }
+  const uint32_t slot = wv * 4u + qd;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    s_mx[slot][c0 + i] = mx[i];
+    s_mn[slot][c0 + i] = mn[i];
+    s_q[slot][c0 + i] = q[i];
+    s_ds[slot][c0 + i] = ds[i];
+  }
+  __syncthreads();
+  if (wv == 0) {
+    const uint32_t j = l;
+    float fx = s_mx[0][j], fn = s_mn[0][j];
+    uint32_t fq = s_q[0][j];
+    unsigned long long fd = s_ds[0][j];
+#pragma unroll
+    for (int w = 1; w < kQuarters; ++w) {
+      fx = fmaxf(fx, s_mx[w][j]);
+      fn = fminf(fn, s_mn[w][j]);
+      fq += s_q[w][j];
+      fd += s_ds[w][j];
+    }
+    const uint32_t col = b * kBlkCols + j;
+    if (fq && A.active[col]) {
+      atomicMax(&A.st_mx[col], __float_as_uint(fx));
+      atomicMin(&A.st_mn[col], __float_as_uint(fn));
+      atomicAdd(&A.st_q[col], fq);
+      atomicAdd(&A.st_dsum[col], fd);
     }
   }
 }
@@ -701,7 +861,12 @@ int launch_block_subset(tgtp_ctx* ctx, hipStream_t s, const BlockArgs& A, const 
     float* nxt = (r & 1u) ? A.xb : A.xa;
     const uint32_t tr = round_tiles[r];
     if (tr == 0) continue;
-    blk_round_kernel<<<grid_for(tr), 256, 0, s>>>(A, S.b0, S.b1, tr, cur, nxt);
+    if (A.round_form == 1)
+      blk_round4_kernel<8, 6><<<grid_for(tr), 256, 0, s>>>(A, S.b0, S.b1, tr, cur, nxt);
+    else if (A.round_form == 2)
+      blk_round4_kernel<16, 4><<<grid_for(tr), 256, 0, s>>>(A, S.b0, S.b1, tr, cur, nxt);
+    else
+      blk_round_kernel<<<grid_for(tr), 256, 0, s>>>(A, S.b0, S.b1, tr, cur, nxt);
     blk_stop_kernel<<<col_blocks, 256, 0, s>>>(A, S.b0, S.b1, nxt);
   }
   blk_transpose_kernel<<<tile_grid, 256, 0, s>>>(A, S.b0, S.b1, S.tiles);

@@ -62,6 +62,7 @@ struct BlockArgs {
   uint32_t tile_mode;  // 0: per-XCD runs of tiles; 1: per-block groups of 8 tiles
   uint32_t light;      // degree up to which one thread scans a vertex's row
   uint32_t small_bq;   // walk positions with at most this many bouquets: one 8-lane group
+  uint32_t round_form; // 0: one row per wave; 1 / 2: four rows per wave, 8 / 16 gathers per step
   int32_t* tr_ell;
   float* tr_emax;
   float* tr_gmax;

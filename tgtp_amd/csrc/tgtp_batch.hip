@@ -93,6 +93,7 @@ struct FastPlan {
   float* d_out = nullptr;
   uint32_t max_slots = 0, slots = 0;  // resident workgroups of the post kernel
   uint32_t tile_mode = 0;
+  uint32_t round_form = 0;  // blk_round_kernel (0) or the four-rows-per-wave form (1, 2)
   // tuned parameters
   uint32_t light = 32, small_bq = 8;
   // captured run
@@ -317,6 +318,14 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
   P->tile_mode = static_cast<uint64_t>(n_max) * kBlkCols * 4 > (16ull << 20) ? 1u : 0u;
   const int tm = env_int_batch("TGTP_FAST_TILE_MODE", -1);
   if (tm == 0 || tm == 1) P->tile_mode = static_cast<uint32_t>(tm);
+  // diffusion rounds four rows per wave: 16 gathers per step at 4 waves per
+  // SIMD where rows are long (mean degree >= 8), 8 per step at 6 waves per
+  // SIMD on short rows (grids); TGTP_FAST_ROUND=0 selects one row per wave
+  P->round_form = dg->m >= 8ull * dg->n ? 2u : 1u;
+  const int rf = env_int_batch("TGTP_FAST_ROUND", -1);
+  if (rf >= 0 && rf <= 2) P->round_form = static_cast<uint32_t>(rf);
+  // the four-rows form addresses a block's state with 32-bit buffer offsets
+  if (n_max >= (1u << 24)) P->round_form = 0;
   auto units_of = [&](uint32_t n) {
     const uint32_t t = (n + kBlkRows - 1) / kBlkRows;
     return P->tile_mode == 0 ? t : (t + 7) / 8;
@@ -421,6 +430,7 @@ BlockArgs plan_args(const FastPlan* P, tgtp_dgraph* dg) {
   A.light = P->light;
   A.small_bq = P->small_bq;
   A.tile_mode = P->tile_mode;
+  A.round_form = P->round_form;
   return A;
 }
 
