@@ -35,6 +35,7 @@ constexpr int kPB = 256;               // post kernel: threads per source
 constexpr int kPW = kPB / 64;
 constexpr uint32_t kPostLds = 12288;   // floats of a source vector staged in LDS
 constexpr uint32_t kTopK = 16;         // single-pass top-gamma for gamma <= 16
+constexpr uint32_t kPassU = 8;         // loads per thread in flight in passes over a vector
 constexpr double kMaxWalksFast = 68719476736.0;
 constexpr uint32_t kInfBits = 0x7f800000u;
 
@@ -505,6 +506,7 @@ struct PostShared {
   uint32_t k;
   uint32_t cand[kCandCap];
   uint32_t n_cand;
+  uint32_t list_all;
   double wacc[kPB / 8];   // per 8-lane group walk sums
 };
 
@@ -552,22 +554,46 @@ __device__ __forceinline__ double group_sum8(float x, uint32_t grp_base) {
 }
 
 // gamma-th largest value of the frontier (pv > 0), counting ties; -inf when
-// the frontier is empty.  cand: kPB * kTopK floats of LDS.
+// the frontier is empty.  cand: kPB * kTopK floats of LDS.  For rg <= kTopK
+// each thread's kTopK largest values (of the vertices it visits) and their
+// vertices are left in top / tix, largest first.
 __device__ float top_gamma_value(const float* pv, uint32_t n, uint32_t rg, float* cand,
-                                 PostShared& sh) {
+                                 PostShared& sh, float (&top)[kTopK], uint32_t (&tix)[kTopK]) {
   float thr = INFINITY, vstar = -INFINITY;
   uint32_t above = 0;
   if (rg <= kTopK) {
-    float top[kTopK];
 #pragma unroll
-    for (uint32_t i = 0; i < kTopK; ++i) top[i] = -INFINITY;
-    for (uint32_t u = threadIdx.x; u < n; u += kPB) {
-      const float x = pv[u];
-      if (x > 0.0f && x > top[kTopK - 1]) {
+    for (uint32_t i = 0; i < kTopK; ++i) {
+      top[i] = -INFINITY;
+      tix[i] = 0;
+    }
+    // kPassU values per thread loaded before any is inserted: on vectors
+    // read from global memory the loads overlap (the insertion's branch
+    // would otherwise wait for each); the top-16 multiset does not depend
+    // on the order of insertion
+    for (uint32_t u0 = threadIdx.x; u0 < n; u0 += kPB * kPassU) {
+      float xs[kPassU];
 #pragma unroll
-        for (uint32_t i = kTopK - 1; i > 0; --i)
-          top[i] = x > top[i - 1] ? top[i - 1] : (x > top[i] ? x : top[i]);
-        top[0] = x > top[0] ? x : top[0];
+      for (uint32_t c = 0; c < kPassU; ++c) {
+        const uint32_t u = u0 + c * kPB;
+        xs[c] = u < n ? pv[u] : 0.0f;
+      }
+#pragma unroll
+      for (uint32_t c = 0; c < kPassU; ++c) {
+        const float x = xs[c];
+        const uint32_t u = u0 + c * kPB;
+        if (x > 0.0f && x > top[kTopK - 1]) {
+#pragma unroll
+          for (uint32_t i = kTopK - 1; i > 0; --i) {
+            const bool up = x > top[i - 1], here = x > top[i];
+            tix[i] = up ? tix[i - 1] : (here ? u : tix[i]);
+            top[i] = up ? top[i - 1] : (here ? x : top[i]);
+          }
+          if (x > top[0]) {
+            top[0] = x;
+            tix[0] = u;
+          }
+        }
       }
     }
 #pragma unroll
@@ -624,20 +650,57 @@ __device__ void post_source(const BlockArgs& A, const BlockInfo& B, uint32_t col
 
   // ---- top-gamma bound (aesc.h:414-442)
   const uint32_t rg = min(static_cast<uint32_t>(gamma), Q);
-  const float vstar = top_gamma_value(pv, n, rg, cand, sh);
+  float top[kTopK];
+  uint32_t tix[kTopK];
+  const float vstar = top_gamma_value(pv, n, rg, cand, sh, top, tix);
   const float gamma_max = gamma < static_cast<int>(Q) ? fminf(1.0f, vstar) : 0.0f;
   float emax = gmax + gamma_max;
   {
-    // the top-gamma set with its ties: listed in one pass, then one wave
-    // per member scans its row; an overlong tie set takes the chunked scan
+    // the top-gamma set with its ties: listed from the threads' largest
+    // values (or by a pass over the vector), then one wave per member scans
+    // its row; an overlong tie set takes the chunked scan
     float best = -INFINITY;
-    if (threadIdx.x == 0) sh.n_cand = 0;
+    if (threadIdx.x == 0) {
+      sh.n_cand = 0;
+      sh.list_all = rg > kTopK ? 1u : 0u;
+    }
     __syncthreads();
-    for (uint32_t u = threadIdx.x; u < n; u += kPB) {
-      const float pu = pv[u];
-      if (pu > 0.0f && pu >= vstar) {
-        const uint32_t i = atomicAdd(&sh.n_cand, 1u);
-        if (i < kCandCap) sh.cand[i] = u;
+    // a thread's members are among its kTopK largest values unless the
+    // smallest of those is itself a member (then ties may hide more): only
+    // then is the vector scanned again
+    if (rg <= kTopK) {
+      if (top[kTopK - 1] > 0.0f && top[kTopK - 1] >= vstar) {
+        sh.list_all = 1u;
+      } else {
+#pragma unroll
+        for (uint32_t i = 0; i < kTopK; ++i) {
+          if (top[i] > 0.0f && top[i] >= vstar) {
+            const uint32_t at = atomicAdd(&sh.n_cand, 1u);
+            if (at < kCandCap) sh.cand[at] = tix[i];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    const bool rescan = sh.list_all != 0u;
+    if (rescan) {
+      __syncthreads();
+      if (threadIdx.x == 0) sh.n_cand = 0;
+      __syncthreads();
+    }
+    for (uint32_t u0 = threadIdx.x; rescan && u0 < n; u0 += kPB * kPassU) {
+      float xs[kPassU];
+#pragma unroll
+      for (uint32_t c = 0; c < kPassU; ++c) {
+        const uint32_t u = u0 + c * kPB;
+        xs[c] = u < n ? pv[u] : 0.0f;
+      }
+#pragma unroll
+      for (uint32_t c = 0; c < kPassU; ++c) {
+        if (xs[c] > 0.0f && xs[c] >= vstar) {
+          const uint32_t i = atomicAdd(&sh.n_cand, 1u);
+          if (i < kCandCap) sh.cand[i] = u0 + c * kPB;
+        }
       }
     }
     __syncthreads();
@@ -806,7 +869,10 @@ __device__ void post_source(const BlockArgs& A, const BlockInfo& B, uint32_t col
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(kPB) blk_post_kernel(BlockArgs A, uint32_t b0, uint32_t b1) {
+// Vectors of blocks with n <= stage_max (0 or kPostLds, the dynamic LDS the
+// launch provides) are staged in LDS; larger ones are read in place.
+__global__ void __launch_bounds__(kPB) blk_post_kernel(BlockArgs A, uint32_t b0, uint32_t b1,
+                                                       uint32_t stage_max) {
   extern __shared__ float lds_pv[];
   __shared__ float cand[kPB * kTopK];
   __shared__ PostShared sh;
@@ -823,8 +889,18 @@ __global__ void __launch_bounds__(kPB) blk_post_kernel(BlockArgs A, uint32_t b0,
     const uint32_t L = A.last[b];
     const float* O = ((L & 1u) ? A.xa : A.xb) + B.xoff + static_cast<uint64_t>(j) * B.n;
     const float* pv = O;
-    if (B.n <= kPostLds) {
-      for (uint32_t u = threadIdx.x; u < B.n; u += kPB) lds_pv[u] = O[u];
+    if (B.n <= stage_max) {
+      for (uint32_t u0 = threadIdx.x; u0 < B.n; u0 += kPB * kPassU) {
+        float xs[kPassU];
+#pragma unroll
+        for (uint32_t c = 0; c < kPassU; ++c) {
+          const uint32_t u = u0 + c * kPB;
+          xs[c] = u < B.n ? O[u] : 0.0f;
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < kPassU; ++c)
+          if (u0 + c * kPB < B.n) lds_pv[u0 + c * kPB] = xs[c];
+      }
       __syncthreads();
       pv = lds_pv;
     }
@@ -834,17 +910,18 @@ __global__ void __launch_bounds__(kPB) blk_post_kernel(BlockArgs A, uint32_t b0,
 
 }  // namespace
 
-int block_post_per_cu() {
+int block_post_per_cu(bool staged) {
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
           &per_cu, reinterpret_cast<const void*>(&blk_post_kernel), kPB,
-          kPostLds * sizeof(float)) != hipSuccess)
+          staged ? kPostLds * sizeof(float) : 0) != hipSuccess)
     per_cu = 1;
   return std::max(1, per_cu);
 }
 
 int launch_block_subset(tgtp_ctx* ctx, hipStream_t s, const BlockArgs& A, const BlockSubset& S,
-                        const uint32_t* round_tiles, uint32_t post_slots) {
+                        const uint32_t* round_tiles, uint32_t post_slots,
+                        uint32_t post_slots_plain) {
   const uint32_t nb = S.b1 - S.b0;
   const uint32_t col_blocks = (nb * kBlkCols + 3) / 4;
   const uint64_t zb = std::min<uint64_t>(16384, (S.floats + 255) / 256 + 1);
@@ -871,8 +948,13 @@ int launch_block_subset(tgtp_ctx* ctx, hipStream_t s, const BlockArgs& A, const 
   }
   blk_transpose_kernel<<<tile_grid, 256, 0, s>>>(A, S.b0, S.b1, S.tiles);
   blk_zero_kernel<<<1, 256, 0, s>>>(reinterpret_cast<float*>(A.next_src), 1);
-  blk_post_kernel<<<std::max(1u, std::min(post_slots, nb * kBlkCols)), kPB,
-                    kPostLds * sizeof(float), s>>>(A, S.b0, S.b1);
+  // a subset without vectors to stage runs without the LDS buffer, so more
+  // workgroups share a CU
+  const bool staged = S.n_min <= kPostLds;
+  const uint32_t slots = staged ? post_slots : post_slots_plain;
+  blk_post_kernel<<<std::max(1u, std::min(slots, nb * kBlkCols)), kPB,
+                    staged ? kPostLds * sizeof(float) : 0, s>>>(A, S.b0, S.b1,
+                                                               staged ? kPostLds : 0u);
   TGTP_HIP_CHECK(ctx, hipGetLastError());
   return TGTP_OK;
 }

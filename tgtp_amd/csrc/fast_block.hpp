@@ -77,12 +77,17 @@ struct BlockSubset {
   uint32_t rounds;     // rounds to launch (>= every source's stopping round)
   uint32_t round_tiles0;  // index of this subset's per-round tile counts
   uint64_t floats;     // state floats per buffer
+  uint32_t n_min;      // vertices of the subset's smallest graph
 };
 
 // Enqueues the whole per-source phase for one subset on `stream`.
 // round_tiles[r]: the tile units round r covers (the subset's blocks are
 // ordered by decreasing round bound, so they are a prefix).
+// post_slots / post_slots_plain: post kernel workgroups for subsets with /
+// without vectors staged in LDS.
 int launch_block_subset(tgtp_ctx* ctx, hipStream_t stream, const BlockArgs& A,
-                        const BlockSubset& S, const uint32_t* round_tiles, uint32_t post_slots);
-// Resident workgroups per CU of the post kernel.
-int block_post_per_cu();
+                        const BlockSubset& S, const uint32_t* round_tiles, uint32_t post_slots,
+                        uint32_t post_slots_plain);
+// Resident workgroups per CU of the post kernel, launched with (staged) or
+// without its LDS vector buffer.
+int block_post_per_cu(bool staged);

@@ -92,6 +92,7 @@ struct FastPlan {
   uint32_t* d_words = nullptr;
   float* d_out = nullptr;
   uint32_t max_slots = 0, slots = 0;  // resident workgroups of the post kernel
+  uint32_t per_cu_staged = 1, per_cu_plain = 1;  // ... per CU, with / without its LDS buffer
   uint32_t tile_mode = 0;
   uint32_t round_form = 0;  // blk_round_kernel (0) or the four-rows-per-wave form (1, 2)
   // tuned parameters
@@ -369,6 +370,8 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
       rounds = std::max(rounds, br[i]);
     }
     S.b1 = b;
+    S.n_min = ~0u;
+    for (const BlockInfo& x : bs) S.n_min = std::min(S.n_min, x.n);
     S.tiles = tiles;
     S.rounds = rounds;
     S.floats = fl;
@@ -393,7 +396,9 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
       (rc = dev_alloc(ctx, &P->d_colu, 4 * cols + P->n_blocks, "column statistics")) ||
       (rc = dev_alloc(ctx, &P->d_dsum, cols, "column degree sums")))
     return fail(rc);
-  P->max_slots = static_cast<uint32_t>(static_cast<uint64_t>(ctx->num_cus) * block_post_per_cu());
+  P->per_cu_staged = static_cast<uint32_t>(block_post_per_cu(true));
+  P->per_cu_plain = static_cast<uint32_t>(block_post_per_cu(false));
+  P->max_slots = static_cast<uint32_t>(static_cast<uint64_t>(ctx->num_cus) * P->per_cu_staged);
   P->slots = P->max_slots;
   TGTP_HIP_CHECK(ctx, hipStreamSynchronize(s));
   *out = P;
@@ -489,7 +494,8 @@ int enqueue_run(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_
     const bool second = dual && (i & 1u);
     hipStream_t st = second ? P->side : ctx->stream;
     if ((rc = launch_block_subset(ctx, st, second ? A2 : A, S,
-                                  P->round_tiles.data() + S.round_tiles0, P->slots)))
+                                  P->round_tiles.data() + S.round_tiles0, P->slots,
+                                  std::max(1u, P->slots * P->per_cu_plain / P->per_cu_staged))))
       return rc;
     if (progress) {
       TGTP_HIP_CHECK(ctx, hipStreamSynchronize(st));
@@ -506,8 +512,8 @@ int enqueue_run(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_
   return launch_symmetrize_d(ctx, P->d_preds, src_of, P->m, P->d_out);
 }
 
-// Second stream and state buffers for a batch plan with more than one
-// subset (TGTP_BATCH_STREAMS=1 keeps one stream).
+// Second stream and state buffers for a plan with more than one subset
+// (TGTP_BATCH_STREAMS=1 keeps one stream).
 int enable_dual(tgtp_ctx* ctx, FastPlan* P) {
   if (P->subsets.size() < 2 || env_int_batch("TGTP_BATCH_STREAMS", 2) < 2) return TGTP_OK;
   int rc;
@@ -607,8 +613,14 @@ int single_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e, const tgtp
   specs[0] = GraphSpec{0u, static_cast<uint32_t>(dg->n), 0ull, dg->m, e->values, p->epsilon,
                        p->delta, p->gamma};
   FastPlan* P = nullptr;
-  const int rc = build_plan(ctx, dg, specs, e->omega, p->mode, &P);
+  int rc = build_plan(ctx, dg, specs, e->omega, p->mode, &P);
   if (rc) return rc;
+  // a graph whose blocks take several subsets alternates them between two
+  // streams as a batch does
+  if ((rc = enable_dual(ctx, P))) {
+    free_plan(P);
+    return rc;
+  }
   prep->fast_plan = P;
   prep->fast_plan_free = [](void* q) { free_plan(static_cast<FastPlan*>(q)); };
   *out = P;
