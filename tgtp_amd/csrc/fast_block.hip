@@ -262,18 +262,15 @@ constexpr uint32_t kOutOfRange = 0xfffffff0u;  // buffer offset past any state
 constexpr int kRsrcWord3 = 0x00020000;         // gfx9 raw buffer: 32-bit data, no swizzle
 
 // kC neighbours k .. k + kC - 1 of the quarter's row added to this lane's
-// four sums: lane ql < kC of the quarter fetches id k + ql, the ids are read
-// across the quarter, then all kC 16-byte gathers are issued before the adds.
-// Slots past the row's degree (and lanes whose four sources have all
-// stopped) read out of range: no memory access, and the zeros they return
-// leave the non-negative sums unchanged.
+// four sums, with lane ql of the quarter holding the id of neighbour k + ql
+// in `my`: the ids are read across the quarter, then all kC 16-byte gathers
+// are issued before the adds.  Slots past the row's degree (and lanes whose
+// four sources have all stopped) read out of range: no memory access, and
+// the zeros they return leave the non-negative sums unchanged.
 template <int kC>
-__device__ __forceinline__ void gather_step(__amdgpu_buffer_rsrc_t xrs, const uint32_t* nb,
-                                            uint32_t voff, uint32_t deg, bool any, uint32_t l,
-                                            uint32_t k, float4& sum) {
+__device__ __forceinline__ void gather_ids(__amdgpu_buffer_rsrc_t xrs, uint32_t my, uint32_t deg,
+                                           bool any, uint32_t l, uint32_t k, float4& sum) {
   const uint32_t ql = l & 15u;
-  uint32_t my = 0;
-  if (ql < static_cast<uint32_t>(kC) && k + ql < deg) my = nb[k + ql] - voff;
   uint32_t off[kC];
 #pragma unroll
   for (int c = 0; c < kC; ++c) {
@@ -291,6 +288,14 @@ __device__ __forceinline__ void gather_step(__amdgpu_buffer_rsrc_t xrs, const ui
     sum.z += v[c].z;
     sum.w += v[c].w;
   }
+}
+
+// Lane ql's neighbour id k + ql of a row (block-local; anything past the
+// row's first 16 from k, or past its degree, reads out of range).
+__device__ __forceinline__ uint32_t row_id(__amdgpu_buffer_rsrc_t ars, uint2 r, uint32_t voff,
+                                           uint32_t ql, uint32_t k) {
+  const uint32_t off = k + ql < r.y ? (r.x + k + ql) * 4u : kOutOfRange;
+  return __builtin_amdgcn_raw_buffer_load_b32(ars, off, 0, 0) - voff;
 }
 
 // The same round with four rows per wave: each 16-lane quarter takes one
@@ -323,57 +328,93 @@ blk_round4_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_units,
   uint32_t q[4] = {0, 0, 0, 0};
   unsigned long long ds[4] = {0, 0, 0, 0};
   const uint32_t rend = min(row0 + kBlkRows, B.n);
-  const float* Xc = cur + B.xoff + c0;
   float* Xn = nxt + B.xoff + c0;
   const uint32_t voff = B.voff;
   // the block's state as a buffer resource (n x 256 bytes; the host keeps
   // this form to n < 2^24): a gather is a 16-byte buffer load at lane offset
-  // id * 256 + 16 * ql
+  // id * 256 + 16 * ql.  The adjacency likewise (m < 2^30).
   const __amdgpu_buffer_rsrc_t xrs = __builtin_amdgcn_make_buffer_rsrc(
       const_cast<float*>(cur + B.xoff), 0, static_cast<int>(B.n * (kBlkCols * 4u)), kRsrcWord3);
-  // quarter qd of wave wv takes rows row0 + 16 i + 4 wv + qd: a wave's four
-  // rows are consecutive (similar degrees) and its stores one 1 KB run
-  for (uint32_t base = row0 + 4u * __builtin_amdgcn_readfirstlane(wv); base < rend; base += 16u) {
+  const __amdgpu_buffer_rsrc_t ars = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<uint32_t*>(A.adj), 0, static_cast<int>(A.adj_words * 4u), kRsrcWord3);
+  // quarter qd of wave wv takes rows row0 + 16 g + 4 wv + qd, g = 0..3: a
+  // wave's four rows are consecutive (similar degrees) and its stores one
+  // 1 KB run.  Software pipeline over g: the four row descriptors load
+  // first, and the next group's first 16 neighbour ids load while this
+  // group's gathers are in flight.
+  constexpr uint32_t kGroups = kBlkRows / 16u;
+  const uint32_t base0 = row0 + 4u * __builtin_amdgcn_readfirstlane(wv);
+  uint2 rr[kGroups];
+  {
+    // buffer loads (rows past the tile read out of range as {0, 0}): no
+    // branches, so all four are in flight at once
+    const __amdgpu_buffer_rsrc_t rrs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint2*>(A.rowd + voff), 0, static_cast<int>(B.n * 8u), kRsrcWord3);
+#pragma unroll
+    for (uint32_t g = 0; g < kGroups; ++g) {
+      const uint32_t u = base0 + 16u * g + qd;
+      rr[g] = __builtin_bit_cast(
+          uint2, __builtin_amdgcn_raw_buffer_load_b64(rrs, u < rend ? u * 8u : kOutOfRange, 0, 0));
+    }
+  }
+  uint32_t my_next = row_id(ars, rr[0], voff, ql, 0);
+#pragma unroll
+  for (uint32_t g = 0; g < kGroups; ++g) {
+    const uint32_t base = base0 + 16u * g;
+    if (base >= rend) break;  // uniform
     const uint32_t u = base + qd;
     const bool rv = u < rend;
-    uint2 r = make_uint2(0u, 0u);
-    if (rv) r = A.rowd[voff + u];
+    const uint2 r = rr[g];
+    const uint32_t my0 = my_next;
+    if (g + 1 < kGroups) my_next = row_id(ars, rr[g + 1], voff, ql, 0);
     // the row's degree is the same in all 16 lanes of the quarter, whatever
     // their sources' flags: every lane fetches its share of the ids
     const uint32_t deg = r.y;
     const uint32_t dmax = max(max(__builtin_amdgcn_readlane(deg, 0), __builtin_amdgcn_readlane(deg, 16)),
                               max(__builtin_amdgcn_readlane(deg, 32), __builtin_amdgcn_readlane(deg, 48)));
-    const uint32_t* nb = A.adj + r.x;
+    const uint64_t ro = static_cast<uint64_t>(u) * kBlkCols;
+    // stopped sources keep their value: read with the gathers (out of range,
+    // so no access, when all four of the lane's sources are active)
+    const float4 old = __builtin_bit_cast(
+        float4, __builtin_amdgcn_raw_buffer_load_b128(
+                    xrs, rv && !all ? static_cast<uint32_t>(ro) * 4u + ql * 16u : kOutOfRange, 0, 0));
     float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     // steps of kChunk gathers while a row of the wave has more than half a
-    // chunk left, then of 8 and 4 (fewer idle slots on short rows)
-    for (uint32_t k = 0; k < dmax;) {
+    // chunk left, then of 8 and 4 (fewer idle slots on short rows); the
+    // first step's ids are the prefetched ones
+    uint32_t k = 0;
+    if (kChunk >= 16 && dmax > 8) {
+      gather_ids<16>(xrs, my0, deg, any, l, 0, sum);
+      k = 16;
+    } else if (dmax > 4) {
+      gather_ids<8>(xrs, my0, deg, any, l, 0, sum);
+      k = 8;
+    } else if (dmax > 0) {
+      gather_ids<4>(xrs, my0, deg, any, l, 0, sum);
+      k = 4;
+    }
+    while (k < dmax) {
       const uint32_t rem = dmax - k;
+      const uint32_t my = row_id(ars, r, voff, ql, k);
       if (kChunk >= 16 && rem > 8) {
-        gather_step<16>(xrs, nb, voff, deg, any, l, k, sum);
+        gather_ids<16>(xrs, my, deg, any, l, k, sum);
         k += 16;
       } else if (rem > 4) {
-        gather_step<8>(xrs, nb, voff, deg, any, l, k, sum);
+        gather_ids<8>(xrs, my, deg, any, l, k, sum);
         k += 8;
       } else {
-        gather_step<4>(xrs, nb, voff, deg, any, l, k, sum);
+        gather_ids<4>(xrs, my, deg, any, l, k, sum);
         k += 4;
       }
     }
     if (rv) {
-      const uint64_t ro = static_cast<uint64_t>(u) * kBlkCols;
-      float4 val;
-      if (any) {
-        const float d = A.degf[voff + u];
-        val = make_float4(sum.x / d, sum.y / d, sum.z / d, sum.w / d);
-      }
-      if (!all) {
-        const float4 old = *reinterpret_cast<const float4*>(Xc + ro);
-        if (!act[0]) val.x = old.x;
-        if (!act[1]) val.y = old.y;
-        if (!act[2]) val.z = old.z;
-        if (!act[3]) val.w = old.w;
-      }
+      // the degree as a float, as the graph's degf table holds it
+      const float d = static_cast<float>(deg);
+      float4 val = make_float4(sum.x / d, sum.y / d, sum.z / d, sum.w / d);
+      if (!act[0]) val.x = old.x;
+      if (!act[1]) val.y = old.y;
+      if (!act[2]) val.z = old.z;
+      if (!act[3]) val.w = old.w;
       *reinterpret_cast<float4*>(Xn + ro) = val;
       const float vv[4] = {val.x, val.y, val.z, val.w};
 #pragma unroll

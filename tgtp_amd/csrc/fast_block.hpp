@@ -63,6 +63,7 @@ struct BlockArgs {
   uint32_t light;      // degree up to which one thread scans a vertex's row
   uint32_t small_bq;   // walk positions with at most this many bouquets: one 8-lane group
   uint32_t round_form; // 0: one row per wave; 1 / 2: four rows per wave, 8 / 16 gathers per step
+  uint32_t adj_words;  // CSR entries (the four-rows form needs m < 2^30)
   int32_t* tr_ell;
   float* tr_emax;
   float* tr_gmax;

@@ -325,8 +325,9 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
   P->round_form = dg->m >= 8ull * dg->n ? 2u : 1u;
   const int rf = env_int_batch("TGTP_FAST_ROUND", -1);
   if (rf >= 0 && rf <= 2) P->round_form = static_cast<uint32_t>(rf);
-  // the four-rows form addresses a block's state with 32-bit buffer offsets
-  if (n_max >= (1u << 24)) P->round_form = 0;
+  // the four-rows form addresses a block's state and the adjacency with
+  // 32-bit buffer offsets
+  if (n_max >= (1u << 24) || dg->m >= (1ull << 30)) P->round_form = 0;
   auto units_of = [&](uint32_t n) {
     const uint32_t t = (n + kBlkRows - 1) / kBlkRows;
     return P->tile_mode == 0 ? t : (t + 7) / 8;
@@ -436,6 +437,7 @@ BlockArgs plan_args(const FastPlan* P, tgtp_dgraph* dg) {
   A.small_bq = P->small_bq;
   A.tile_mode = P->tile_mode;
   A.round_form = P->round_form;
+  A.adj_words = static_cast<uint32_t>(std::min<uint64_t>(dg->m, 0xffffffffull));
   return A;
 }
 
