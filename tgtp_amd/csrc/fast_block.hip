@@ -382,30 +382,25 @@ blk_round4_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_units,
     // steps of kChunk gathers while a row of the wave has more than half a
     // chunk left, then of 8 and 4 (fewer idle slots on short rows); the
     // first step's ids are the prefetched ones
-    uint32_t k = 0;
-    if (kChunk >= 16 && dmax > 8) {
-      gather_ids<16>(xrs, my0, deg, any, l, 0, sum);
-      k = 16;
-    } else if (dmax > 4) {
-      gather_ids<8>(xrs, my0, deg, any, l, 0, sum);
-      k = 8;
-    } else if (dmax > 0) {
-      gather_ids<4>(xrs, my0, deg, any, l, 0, sum);
-      k = 4;
-    }
-    while (k < dmax) {
+    // (each step's ids load during the previous step's gathers)
+    auto step_of = [&](uint32_t k) -> uint32_t {
       const uint32_t rem = dmax - k;
-      const uint32_t my = row_id(ars, r, voff, ql, k);
-      if (kChunk >= 16 && rem > 8) {
+      return kChunk >= 16 && rem > 8 ? 16u : (rem > 4 ? 8u : 4u);
+    };
+    uint32_t k = 0, my = my0;
+    while (k < dmax) {
+      const uint32_t st = step_of(k);
+      const uint32_t kn = k + st;
+      // (unconditional: past the degree it reads out of range)
+      const uint32_t my_n = row_id(ars, r, voff, ql, kn);
+      if (st == 16u)
         gather_ids<16>(xrs, my, deg, any, l, k, sum);
-        k += 16;
-      } else if (rem > 4) {
+      else if (st == 8u)
         gather_ids<8>(xrs, my, deg, any, l, k, sum);
-        k += 8;
-      } else {
+      else
         gather_ids<4>(xrs, my, deg, any, l, k, sum);
-        k += 4;
-      }
+      k = kn;
+      my = my_n;
     }
     if (rv) {
       // the degree as a float, as the graph's degf table holds it
