@@ -33,7 +33,6 @@ namespace {
 
 constexpr int kPB = 256;               // post kernel: threads per source
 constexpr int kPW = kPB / 64;
-constexpr uint32_t kPostLds = 12288;   // floats of a source vector staged in LDS
 constexpr uint32_t kTopK = 16;         // single-pass top-gamma for gamma <= 16
 constexpr uint32_t kPassU = 8;         // loads per thread in flight in passes over a vector
 constexpr double kMaxWalksFast = 68719476736.0;

@@ -21,6 +21,7 @@
 
 constexpr uint32_t kBlkCols = 64;  // sources per block (one per lane)
 constexpr uint32_t kBlkRows = 64;  // vertices per round tile
+constexpr uint32_t kPostLds = 12288;  // post kernel: vectors up to this size are staged in LDS
 
 struct BlockInfo {
   uint32_t voff;   // first vertex of the block's graph in the union

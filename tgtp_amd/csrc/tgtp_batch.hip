@@ -286,11 +286,16 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
   if (dg->m)
     TGTP_HIP_CHECK(ctx, hipMemcpyAsync(taus.data(), P->d_taus, dg->m * 4, hipMemcpyDeviceToHost, s));
   TGTP_HIP_CHECK(ctx, hipStreamSynchronize(s));
-  // blocks: sources of the largest graphs first
+  // blocks: graphs whose vectors the post kernel reads in place (n >
+  // kPostLds) before those it stages in LDS, so that a subset seldom mixes
+  // the two (a mixed subset takes the staged launch, half as many
+  // workgroups per CU); within each group the largest graphs first
   std::vector<uint32_t> order(G);
   std::iota(order.begin(), order.end(), 0u);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](uint32_t a, uint32_t b) { return specs[a].m > specs[b].m; });
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    const bool la = specs[a].n > kPostLds, lb = specs[b].n > kPostLds;
+    return la != lb ? la : specs[a].m > specs[b].m;
+  });
   std::vector<BlockInfo> blocks;
   std::vector<uint32_t> bround;
   for (uint32_t k : order) {
@@ -332,9 +337,12 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
     const uint32_t t = (n + kBlkRows - 1) / kBlkRows;
     return P->tile_mode == 0 ? t : (t + 7) / 8;
   };
-  // subsets of blocks whose states fit one state buffer
+  // subsets of blocks whose states fit one state buffer (4 GiB by default:
+  // measured against 1, 2, 8 and 16 GiB on the batched workload's large
+  // graphs; larger subsets mean fewer post-kernel tails, beyond 4 GiB the
+  // two streams' working sets crowd the caches)
   const uint64_t budget =
-      static_cast<uint64_t>(std::max(1, env_int_batch("TGTP_FAST_STATE_MIB", 2048))) << 18;  // floats
+      static_cast<uint64_t>(std::max(1, env_int_batch("TGTP_FAST_STATE_MIB", 4096))) << 18;  // floats
   uint64_t max_floats = 0;
   for (uint32_t b0 = 0; b0 < P->n_blocks;) {
     BlockSubset S{};
