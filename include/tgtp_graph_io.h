@@ -35,6 +35,17 @@ typedef struct tgtp_csr {
 int tgtp_load_txt(const char* path, tgtp_csr* out, char* err, size_t errlen);
 void tgtp_csr_free(tgtp_csr* g);
 
+/* Binary CSR cache -- no reference counterpart (the reference parses the
+ * text on every run, graph.h:29-97); for callers that load the same large
+ * graph repeatedly.  Layout, little-endian: "TGTPCSR1", u64 n, u64 m,
+ * u32 idx[n+1], u32 adj[m], u64 digest of the two arrays.  Save writes a
+ * temporary file and renames it: 0, -1 bad argument, -4 write failure.
+ * Load checks the size, the digest and the CSR invariants (idx from 0 to m
+ * non-decreasing, ids < n, rows sorted): 0, -1 damaged or foreign file,
+ * -3 out of host memory, -4 cannot open. */
+int tgtp_save_csr_bin(const tgtp_csr* g, const char* path);
+int tgtp_load_csr_bin(const char* path, tgtp_csr* out, char* err, size_t errlen);
+
 /* aesc.h:51-64: omega lines "lambda x_0 ... x_{n-1}", vector entries parsed
  * directly as float; non-finite values are an error (-1), missing file -4.
  * values [omega] double, vectors [n*omega] float vertex-major. */

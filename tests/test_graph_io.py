@@ -129,3 +129,89 @@ def test_reverse_index_is_first_occurrence(tmp_path):
         v = g.adj[pos]
         row = g.adj[g.idx[v]:g.idx[v + 1]]
         assert rev[pos] == g.idx[v] + int(np.searchsorted(row, u[pos], side="left"))
+
+
+def test_loader_parallel_chunks_match_oracle(tmp_path):
+    """A text graph of several MiB is parsed by all threads in line-aligned
+    chunks; the CSR equals the oracle's sequential loader, comments, blank
+    lines, extra fields, duplicates and self-loops included."""
+    rng = np.random.default_rng(7)
+    k = 400_000
+    a = rng.integers(0, 50_000, k)
+    b = rng.integers(0, 50_000, k)
+    lines = []
+    for i in range(k):
+        if i % 9973 == 0:
+            lines.append("# comment line")
+        if i % 7919 == 0:
+            lines.append("   ")
+        lines.append(f"{a[i]} {b[i]}" + (" 1.0 extra" if i % 5 == 0 else ""))
+    lines.append("123 123")
+    p = tmp_path / "big.txt"
+    p.write_text("\n".join(lines) + "\n")
+    assert p.stat().st_size > 4 << 20
+    g = _both(p)
+    assert g.m == 2 * (k + 1)
+
+
+def test_loader_reports_first_invalid_line_across_chunks(tmp_path):
+    """Invalid lines in several chunks: the message names the first one in
+    file order, as a sequential reader would."""
+    body = "".join(f"{i % 1000} {(i * 7) % 1000}\n" for i in range(600_000))
+    p = tmp_path / "bad.txt"
+    p.write_text(body + "7 x-first\n" + body + "9 y-second\n" + body)
+    with pytest.raises(ValueError, match="x-first"):
+        G.load_txt(p)
+
+
+def test_csr_binary_cache_round_trip_and_damage(tmp_path):
+    g = W.G.chung_lu(3000, 2.5, 12.0, seed=5)
+    p = tmp_path / "g.csr"
+    G.save_bin(g, p)
+    h = G.load_bin(p)
+    assert np.array_equal(g.idx, h.idx) and np.array_equal(g.adj, h.adj)
+    raw = bytearray(p.read_bytes())
+    for pos in (30, len(raw) // 2, len(raw) - 3):  # idx, adj, digest
+        bad = bytearray(raw)
+        bad[pos] ^= 0x10
+        q = tmp_path / f"bad{pos}.csr"
+        q.write_bytes(bytes(bad))
+        with pytest.raises(ValueError, match="not a valid CSR cache"):
+            G.load_bin(q)
+    q = tmp_path / "short.csr"
+    q.write_bytes(bytes(raw[:-1]))
+    with pytest.raises(ValueError):
+        G.load_bin(q)
+    q = tmp_path / "magic.csr"
+    q.write_bytes(b"XGTPCSR1" + bytes(raw[8:]))
+    with pytest.raises(ValueError):
+        G.load_bin(q)
+    with pytest.raises(OSError):
+        G.load_bin(tmp_path / "missing.csr")
+
+
+def test_csr_binary_cache_checks_invariants(tmp_path):
+    """A file whose digest matches but whose rows are unsorted (written
+    through the same writer) is refused."""
+    g = G.Graph.from_edges(4, [(0, 1), (0, 2), (2, 3)])
+    adj = g.adj.copy()
+    adj[0], adj[1] = adj[1], adj[0]  # row 0 = [2, 1]
+    G.save_bin(G.Graph(g.idx, adj), tmp_path / "u.csr")
+    with pytest.raises(ValueError, match="invariants"):
+        G.load_bin(tmp_path / "u.csr")
+
+
+def test_load_txt_cached(tmp_path):
+    p = tmp_path / "g.txt"
+    p.write_text("0 1\n1 2\n2 0\n")
+    g1 = G.load_txt_cached(p)
+    cache = tmp_path / "g.txt.csr"
+    assert cache.exists()
+    g2 = G.load_txt_cached(p)
+    assert np.array_equal(g1.adj, g2.adj) and np.array_equal(g1.idx, g2.idx)
+    cache.write_bytes(b"junk")
+    import os
+    os.utime(cache, (p.stat().st_mtime + 10, p.stat().st_mtime + 10))
+    g3 = G.load_txt_cached(p)  # damaged cache: parsed again and rewritten
+    assert np.array_equal(g1.adj, g3.adj)
+    assert G.load_bin(cache).m == g1.m

@@ -71,7 +71,8 @@ EXPORTED = [
     "tgtp_score", "tgtp_score_dev", "tgtp_score_trace",
     "tgtp_batch_create", "tgtp_batch_run", "tgtp_batch_destroy", "tgtp_batch_config",
     "tgtp_score_batch",
-    "tgtp_load_txt", "tgtp_csr_free", "tgtp_load_eigens_txt", "tgtp_check_aesc_graph",
+    "tgtp_load_txt", "tgtp_csr_free", "tgtp_save_csr_bin", "tgtp_load_csr_bin",
+    "tgtp_load_eigens_txt", "tgtp_check_aesc_graph",
     "tgtp_gen_chung_lu", "tgtp_gen_grid_rewired", "tgtp_write_graph_txt",
     "tgtp_csr_reverse_index",
 ]

@@ -3,6 +3,7 @@
 #include "tgtp_graph_io.h"
 
 #include <fcntl.h>
+#include <omp.h>
 #include <sys/mman.h>
 #include <sys/stat.h>
 #include <unistd.h>
@@ -78,6 +79,104 @@ struct Mapped {
     return true;
   }
 };
+
+// One chunk of a text graph: its edges in file order, 1 + its largest id,
+// and its first invalid line, if any (parsing stops there).
+struct ParseChunk {
+  std::vector<std::pair<uint32_t, uint32_t>> edges;
+  uint64_t n = 0;
+  const char* bad_line = nullptr;
+  const char* bad_eol = nullptr;
+};
+
+// graph.h:29-97 line rules: '#' / '%' comment lines (also indented), blank
+// lines skipped, two non-negative ids then anything.
+void parse_lines(const char* p, const char* end, ParseChunk& c) {
+  while (p < end) {
+    const char* eol = static_cast<const char*>(std::memchr(p, '\n', static_cast<size_t>(end - p)));
+    if (!eol) eol = end;
+    const char* line = p;
+    p = eol + 1;
+    if (line < eol && (*line == '#' || *line == '%')) continue;
+    const char* q = line;
+    while (q < eol && is_space(*q)) ++q;
+    if (q == eol || *q == '#' || *q == '%') continue;
+    int64_t a, b;
+    if (!parse_i64(q, eol, a) || !parse_i64(q, eol, b) || a < 0 || b < 0 ||
+        a >= 0xFFFFFFFFll || b >= 0xFFFFFFFFll) {
+      c.bad_line = line;
+      c.bad_eol = eol;
+      return;
+    }
+    c.edges.emplace_back(static_cast<uint32_t>(a), static_cast<uint32_t>(b));
+    c.n = std::max<uint64_t>(c.n, static_cast<uint64_t>(std::max(a, b)) + 1);
+  }
+}
+
+// Sorted-row CSR of both directions of every pair of every chunk: degree
+// counts and row fills with atomics across threads, then each row sorted
+// (the order of the fills does not matter after the sort).
+int build_csr_chunks(uint64_t n, const std::vector<ParseChunk>& chunks, tgtp_csr* out) {
+  uint64_t pairs = 0;
+  for (const ParseChunk& c : chunks) pairs += c.edges.size();
+  const uint64_t m = 2ull * pairs;
+  if (m > 0xFFFFFFFFull) return -1;
+  uint32_t* idx = static_cast<uint32_t*>(std::malloc((n + 1) * sizeof(uint32_t)));
+  uint32_t* adj = static_cast<uint32_t*>(std::malloc((m ? m : 1) * sizeof(uint32_t)));
+  if (!idx || !adj) {
+    std::free(idx);
+    std::free(adj);
+    return -3;
+  }
+  std::vector<uint32_t> cnt(n, 0);
+  const int nc = static_cast<int>(chunks.size());
+#pragma omp parallel for schedule(static, 1)
+  for (int t = 0; t < nc; ++t)
+    for (const auto& e : chunks[static_cast<size_t>(t)].edges) {
+      __atomic_fetch_add(&cnt[e.first], 1u, __ATOMIC_RELAXED);
+      __atomic_fetch_add(&cnt[e.second], 1u, __ATOMIC_RELAXED);
+    }
+  idx[0] = 0;
+  for (uint64_t v = 0; v < n; ++v) idx[v + 1] = idx[v] + cnt[v];
+  std::copy(idx, idx + n, cnt.begin());  // next free slot per row
+#pragma omp parallel for schedule(static, 1)
+  for (int t = 0; t < nc; ++t)
+    for (const auto& e : chunks[static_cast<size_t>(t)].edges) {
+      adj[__atomic_fetch_add(&cnt[e.first], 1u, __ATOMIC_RELAXED)] = e.second;
+      adj[__atomic_fetch_add(&cnt[e.second], 1u, __ATOMIC_RELAXED)] = e.first;
+    }
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t v = 0; v < static_cast<int64_t>(n); ++v) std::sort(adj + idx[v], adj + idx[v + 1]);
+  out->n = n;
+  out->m = m;
+  out->idx = idx;
+  out->adj = adj;
+  return 0;
+}
+
+// 64-bit digest of the binary cache's arrays: 8-byte little-endian words
+// (the tail zero-padded), h = rotl(h ^ w, 29) * odd constant, finished with
+// the byte count.
+uint64_t digest_words(const void* p, size_t bytes, uint64_t h) {
+  const unsigned char* c = static_cast<const unsigned char*>(p);
+  size_t i = 0;
+  for (; i + 8 <= bytes; i += 8) {
+    uint64_t w;
+    std::memcpy(&w, c + i, 8);
+    h ^= w;
+    h = ((h << 29) | (h >> 35)) * 0x9e3779b97f4a7c15ull;
+  }
+  if (i < bytes) {
+    uint64_t w = 0;
+    std::memcpy(&w, c + i, bytes - i);
+    h ^= w;
+    h = ((h << 29) | (h >> 35)) * 0x9e3779b97f4a7c15ull;
+  }
+  h ^= bytes;
+  return h * 0xd6e8feb86659fd93ull;
+}
+
+constexpr char kCsrMagic[8] = {'T', 'G', 'T', 'P', 'C', 'S', 'R', '1'};
 
 // Builds a sorted-row CSR holding both directions of each (a, b) pair.
 int build_csr(uint64_t n, const std::vector<std::pair<uint32_t, uint32_t>>& edges,
@@ -194,38 +293,46 @@ extern "C" int tgtp_load_txt(const char* path, tgtp_csr* out, char* err, size_t 
     set_err(err, errlen, std::string("cannot open graph: ") + path);
     return -4;
   }
-  std::vector<std::pair<uint32_t, uint32_t>> edges;
-  uint64_t n = 0;
-  const char* p = f.data;
-  const char* end = f.data + f.size;
-  while (p < end) {
-    const char* eol = static_cast<const char*>(std::memchr(p, '\n', static_cast<size_t>(end - p)));
-    if (!eol) eol = end;
-    const char* line = p;
-    p = eol + 1;
-    if (line < eol && (*line == '#' || *line == '%')) continue;
-    const char* q = line;
-    while (q < eol && is_space(*q)) ++q;
-    if (q == eol || *q == '#' || *q == '%') continue;
-    int64_t a, b;
-    if (!parse_i64(q, eol, a) || !parse_i64(q, eol, b) || a < 0 || b < 0 ||
-        a >= 0xFFFFFFFFll || b >= 0xFFFFFFFFll) {
+  // The file is cut into chunks at line starts, parsed by all threads; a
+  // line belongs to the chunk it starts in, so the first chunk holding an
+  // invalid line holds the file's first invalid line.
+  const char* data = f.data;
+  const size_t size = f.size;
+  const int threads = std::max(1, std::min<int>(omp_get_max_threads(),
+                                                static_cast<int>(size >> 20) + 1));
+  std::vector<const char*> cut(static_cast<size_t>(threads) + 1, data + size);
+  cut[0] = data;
+  for (int t = 1; t < threads; ++t) {
+    const char* c = data + size / static_cast<size_t>(threads) * static_cast<size_t>(t);
+    c = std::max(c, cut[static_cast<size_t>(t) - 1]);
+    const char* nl = c < data + size
+                         ? static_cast<const char*>(std::memchr(c, '\n', static_cast<size_t>(data + size - c)))
+                         : nullptr;
+    cut[static_cast<size_t>(t)] = nl ? nl + 1 : data + size;
+  }
+  std::vector<ParseChunk> chunks(static_cast<size_t>(threads));
+#pragma omp parallel for schedule(static, 1) num_threads(threads)
+  for (int t = 0; t < threads; ++t)
+    parse_lines(cut[static_cast<size_t>(t)], cut[static_cast<size_t>(t) + 1], chunks[static_cast<size_t>(t)]);
+  uint64_t n = 0, pairs = 0;
+  for (const ParseChunk& c : chunks) {
+    if (c.bad_line) {
       set_err(err, errlen, std::string("invalid edge in ") + path + ": " +
-                               std::string(line, static_cast<size_t>(eol - line)));
+                               std::string(c.bad_line, static_cast<size_t>(c.bad_eol - c.bad_line)));
       return -1;
     }
-    edges.emplace_back(static_cast<uint32_t>(a), static_cast<uint32_t>(b));
-    n = std::max<uint64_t>(n, static_cast<uint64_t>(std::max(a, b)) + 1);
+    n = std::max(n, c.n);
+    pairs += c.edges.size();
   }
-  if (edges.empty()) {
+  if (pairs == 0) {
     set_err(err, errlen, std::string("empty graph: ") + path);
     return -1;
   }
-  if (2ull * edges.size() > 0xFFFFFFFFull) {
+  if (2ull * pairs > 0xFFFFFFFFull) {
     set_err(err, errlen, "graph exceeds CSR index capacity");
     return -1;
   }
-  const int rc = build_csr(n, edges, out);
+  const int rc = build_csr_chunks(n, chunks, out);
   if (rc == -3) set_err(err, errlen, "out of host memory");
   return rc;
 }
@@ -237,6 +344,92 @@ extern "C" void tgtp_csr_free(tgtp_csr* g) {
   g->idx = nullptr;
   g->adj = nullptr;
   g->n = g->m = 0;
+}
+
+extern "C" int tgtp_save_csr_bin(const tgtp_csr* g, const char* path) {
+  if (!g || !path || !g->idx || (g->m && !g->adj)) return -1;
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE* fp = std::fopen(tmp.c_str(), "wb");
+  if (!fp) return -4;
+  const uint64_t hdr[2] = {g->n, g->m};
+  const size_t ib = (g->n + 1) * sizeof(uint32_t), ab = g->m * sizeof(uint32_t);
+  uint64_t h = digest_words(g->idx, ib, 0x6a09e667f3bcc908ull ^ g->n);
+  h = digest_words(g->adj, ab, h ^ g->m);
+  bool ok = std::fwrite(kCsrMagic, 1, 8, fp) == 8 && std::fwrite(hdr, 8, 2, fp) == 2 &&
+            std::fwrite(g->idx, 1, ib, fp) == ib && (ab == 0 || std::fwrite(g->adj, 1, ab, fp) == ab) &&
+            std::fwrite(&h, 8, 1, fp) == 1;
+  ok = (std::fclose(fp) == 0) && ok;
+  if (!ok || std::rename(tmp.c_str(), path) != 0) {
+    std::remove(tmp.c_str());
+    return -4;
+  }
+  return 0;
+}
+
+extern "C" int tgtp_load_csr_bin(const char* path, tgtp_csr* out, char* err, size_t errlen) {
+  if (!path || !out) {
+    set_err(err, errlen, "null argument");
+    return -1;
+  }
+  Mapped f;
+  if (!f.open_file(path)) {
+    set_err(err, errlen, std::string("cannot open CSR cache: ") + path);
+    return -4;
+  }
+  const std::string bad = std::string("not a valid CSR cache: ") + path;
+  uint64_t hdr[2];
+  if (f.size < 24 + 4 + 8 || std::memcmp(f.data, kCsrMagic, 8) != 0) {
+    set_err(err, errlen, bad);
+    return -1;
+  }
+  std::memcpy(hdr, f.data + 8, 16);
+  const uint64_t n = hdr[0], m = hdr[1];
+  if (n >= 0xFFFFFFFFull || m > 0xFFFFFFFFull || f.size != 24 + (n + 1) * 4 + m * 4 + 8) {
+    set_err(err, errlen, bad);
+    return -1;
+  }
+  const char* pi = f.data + 24;
+  const char* pa = pi + (n + 1) * 4;
+  uint64_t h = digest_words(pi, (n + 1) * 4, 0x6a09e667f3bcc908ull ^ n);
+  h = digest_words(pa, m * 4, h ^ m);
+  uint64_t stored;
+  std::memcpy(&stored, pa + m * 4, 8);
+  if (h != stored) {
+    set_err(err, errlen, bad + " (checksum)");
+    return -1;
+  }
+  uint32_t* idx = static_cast<uint32_t*>(std::malloc((n + 1) * sizeof(uint32_t)));
+  uint32_t* adj = static_cast<uint32_t*>(std::malloc((m ? m : 1) * sizeof(uint32_t)));
+  if (!idx || !adj) {
+    std::free(idx);
+    std::free(adj);
+    set_err(err, errlen, "out of host memory");
+    return -3;
+  }
+  std::memcpy(idx, pi, (n + 1) * 4);
+  if (m) std::memcpy(adj, pa, m * 4);
+  // CSR invariants: rows in order, ending at m, ids in range, rows sorted
+  bool ok = idx[0] == 0 && idx[n] == m;
+  for (uint64_t v = 0; ok && v < n; ++v) ok = idx[v] <= idx[v + 1];
+  int bad_rows = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : bad_rows)
+  for (int64_t v = 0; v < (ok ? static_cast<int64_t>(n) : 0); ++v)
+    for (uint32_t p = idx[v]; p < idx[v + 1]; ++p)
+      if (adj[p] >= n || (p > idx[v] && adj[p - 1] > adj[p])) {
+        ++bad_rows;
+        break;
+      }
+  if (!ok || bad_rows) {
+    std::free(idx);
+    std::free(adj);
+    set_err(err, errlen, bad + " (CSR invariants)");
+    return -1;
+  }
+  out->n = n;
+  out->m = m;
+  out->idx = idx;
+  out->adj = adj;
+  return 0;
 }
 
 extern "C" int tgtp_load_eigens_txt(const char* path, uint64_t n, int32_t omega,

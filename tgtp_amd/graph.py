@@ -100,6 +100,40 @@ def load_txt(path) -> Graph:
     return _take_csr(cs)
 
 
+def save_bin(g: Graph, path) -> None:
+    """Binary CSR cache (tgtp_save_csr_bin)."""
+    cs = g.csr_struct()
+    if L.lib().tgtp_save_csr_bin(C.byref(cs), str(path).encode()) != 0:
+        raise OSError(f"cannot write {path}")
+
+
+def load_bin(path) -> Graph:
+    """Binary CSR cache (tgtp_load_csr_bin); ValueError for a damaged file."""
+    cs = L.CsrStruct()
+    err = C.create_string_buffer(512)
+    rc = L.lib().tgtp_load_csr_bin(str(path).encode(), C.byref(cs), err, 512)
+    if rc == L.TGTP_E_IO:
+        raise OSError(err.value.decode())
+    if rc != 0:
+        raise ValueError(err.value.decode())
+    return _take_csr(cs)
+
+
+def load_txt_cached(path, cache=None) -> Graph:
+    """load_txt through a binary cache (default: path + ".csr"), rebuilt when
+    missing, older than the text or damaged."""
+    path = Path(path)
+    cache = Path(cache) if cache is not None else path.with_name(path.name + ".csr")
+    if cache.exists() and cache.stat().st_mtime >= path.stat().st_mtime:
+        try:
+            return load_bin(cache)
+        except ValueError:
+            pass
+    g = load_txt(path)
+    save_bin(g, cache)
+    return g
+
+
 def chung_lu(n: int, beta: float, mean_deg: float, w_cap: float = 0.0, seed: int = 1) -> Graph:
     cs = L.CsrStruct()
     rc = L.lib().tgtp_gen_chung_lu(n, beta, mean_deg, w_cap, seed, C.byref(cs))
