@@ -120,6 +120,23 @@ int tgtp_saba_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,
                     uint32_t* counters_out, uint64_t* visit_count,
                     uint64_t* checksum, tgtp_stats* stats);
 
+/* ---- MT19937 comparator walks ----------------------------------------------
+ * The reference's baselines of the walk comparison, on the GPU:
+ *   TGTP_WALK_MT_REGULAR <- scalar_randomwalk + driver   src/rw_experiment.cpp:34-47, 87-108
+ *   TGTP_WALK_MT_SIMD    <- simd_randomwalk + driver     src/rw_experiment_simd.cpp:44-82
+ * Every source s in [src_begin, src_end) with degree > 0 draws from its own
+ * std::mt19937 seeded with 42 + s; n_walks walks of exactly `depth` hops,
+ * next vertex adj[row + draw % degree]; regular: walk i, hop h takes draw
+ * i*depth + h; simd: bouquets of 8 walks, hop h of bouquet b takes draws
+ * b*8*depth + 8h + lane.  Counters and sums as tgtp_saba_walks.  simd needs
+ * n_walks % 8 == 0 (TGTP_E_ARG otherwise); depth up to 3472 (regular) / 434
+ * (simd), TGTP_E_UNSUPPORTED beyond. */
+enum { TGTP_WALK_MT_REGULAR = 0, TGTP_WALK_MT_SIMD = 1 };
+int tgtp_mt_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,
+                  uint32_t src_end, uint32_t depth, uint32_t n_walks, int32_t variant,
+                  uint32_t* counters_out, uint64_t* visit_count, uint64_t* checksum,
+                  tgtp_stats* stats);
+
 /* ---- tau bounds (calculate_taus, aesc.h:164-216) ---------------------------
  * eps_half is float(epsilon/2) as aesc.h:122 passes it.  taus_out [m]. */
 int tgtp_taus(tgtp_ctx* ctx, const tgtp_graph* g, const tgtp_eigens* e,

@@ -43,6 +43,10 @@ def lib() -> C.CDLL:
         L.orc_splitmix32.restype = C.c_uint32
         L.orc_saba_walks.argtypes = [C.c_uint64, u32p, u32p, C.c_uint32, C.c_uint32, C.c_uint32,
                                      C.c_uint64, u32p, C.c_int, C.c_int]
+        L.orc_mt_walks.argtypes = [C.c_uint64, u32p, u32p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                   C.c_uint64, C.c_int, u32p, C.c_int]
+        L.orc_mt19937_nth.argtypes = [C.c_uint32, C.c_uint64]
+        L.orc_mt19937_nth.restype = C.c_uint32
         L.orc_saba_walk_source.argtypes = [u32p, u32p, C.c_uint32, C.c_uint32, C.c_uint64, u32p]
         L.orc_saba_walk_source.restype = None
         L.orc_walk2_saba.argtypes = [u32p, u32p, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32,
@@ -95,6 +99,24 @@ def saba_walks(idx, adj, depth: int, n_walks: int, src_begin: int = 0, src_end=N
     if rc != 0:
         raise ValueError("n_walks must be a multiple of 8 within the 32-bit grid domain")
     return cnt
+
+
+def mt_walks(idx, adj, depth: int, n_walks: int, variant: int = 0, src_begin: int = 0,
+             src_end=None, threads: int = 0) -> np.ndarray:
+    """MT19937 comparator walks (rw_experiment.cpp / rw_experiment_simd.cpp)."""
+    idx, adj = _u32(idx), _u32(adj)
+    n = idx.shape[0] - 1
+    src_end = n if src_end is None else src_end
+    cnt = np.zeros(n, dtype=np.uint32)
+    rc = lib().orc_mt_walks(n, _p(idx, C.c_uint32), _p(adj, C.c_uint32), src_begin, src_end,
+                            depth, n_walks, variant, _p(cnt, C.c_uint32), threads)
+    if rc != 0:
+        raise ValueError("unknown variant, or n_walks not a multiple of 8 for simd")
+    return cnt
+
+
+def mt19937_nth(seed: int, k: int) -> int:
+    return int(lib().orc_mt19937_nth(seed, k))
 
 
 def saba_walk_source(idx, adj, src: int, depth: int, n_walks: int) -> np.ndarray:

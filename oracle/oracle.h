@@ -20,6 +20,14 @@
  *   orc_load_txt / orc_load_eigens / orc_taus / orc_tgtp (walk mode MT19937):
  *       bit-exact against the reference's own aesc_baseline.h and graph.h,
  *       compiled from /root/reference by oracle/Makefile (target ref).
+ *   orc_mt_walks variant 0 (regular): per-vertex counts equal the reference's
+ *       own scalar_randomwalk (rw_experiment.cpp, compiled by oracle/Makefile,
+ *       target ref); the generator is std::mt19937, checked by the standard's
+ *       known answer (orc_mt19937_nth(5489, 10000) == 4123659995).
+ *   orc_mt_walks variant 1 (simd): the reference's simd_randomwalk needs
+ *       Intel SVML (_mm256_rem_epu32), so it is not built; at depth 1 both
+ *       variants draw the same outputs and must agree (tested).  The hop
+ *       order of the draws at depth > 1 is PARITY UNPINNED.
  *   orc_tgtp (walk mode SABA): everything except the 2-way walk is the code
  *       path pinned above.  Phase 1 of the 2-way walk is checked against the
  *       pinned 1-way walk (same grid, hash and multiply-high step).
@@ -56,6 +64,16 @@ int orc_saba_walks(uint64_t n, const uint32_t* idx, const uint32_t* adj,
 void orc_saba_walk_source(const uint32_t* idx, const uint32_t* adj,
                           uint32_t src, uint32_t depth, uint64_t n_walks,
                           uint32_t* counters);
+
+/* MT19937 comparator walks: variant 0 rw_experiment.cpp:34-47, variant 1
+ * rw_experiment_simd.cpp:44-82; per-source std::mt19937 seeded 42 + src
+ * (rw_experiment.cpp:99-104); counters[v] += visits.  -1 for an unknown
+ * variant or (variant 1) n_walks not a multiple of 8. */
+int orc_mt_walks(uint64_t n, const uint32_t* idx, const uint32_t* adj,
+                 uint32_t src_begin, uint32_t src_end, uint32_t depth,
+                 uint64_t n_walks, int variant, uint32_t* counters, int threads);
+/* k-th output (1-based) of std::mt19937(seed). */
+uint32_t orc_mt19937_nth(uint32_t seed, uint64_t k);
 
 /* SABA 2-way walk of aesc.h:241-391; returns the lane-summed float. */
 float orc_walk2_saba(const uint32_t* idx, const uint32_t* adj, uint32_t s,

@@ -175,3 +175,62 @@ float orc_walk2_mt(const uint32_t* idx, const uint32_t* adj, uint32_t s,
   }
   return x;
 }
+
+// ---- MT19937 comparator walks -------------------------------------------
+// std::mt19937 itself (the reference's generator); the C++ standard fixes it
+// ([rand.predef]: the 10000th output of a default-seeded mt19937 is
+// 4123659995), so orc_mt19937_nth checks the library.
+extern "C" uint32_t orc_mt19937_nth(uint32_t seed, uint64_t k) {
+  std::mt19937 rng(seed);
+  uint32_t x = 0;
+  for (uint64_t i = 0; i < k; ++i) x = rng();
+  return x;
+}
+
+// rw_experiment.cpp:34-47 (variant 0: walk i takes draws i*depth .. +depth-1)
+// and rw_experiment_simd.cpp:44-82 (variant 1: bouquets of 8, each hop draws
+// the 8 lanes' numbers in lane order), with the per-source reseeding of
+// rw_experiment.cpp:99-104 and uint32 counters.
+extern "C" int orc_mt_walks(uint64_t n, const uint32_t* idx, const uint32_t* adj,
+                            uint32_t src_begin, uint32_t src_end, uint32_t depth,
+                            uint64_t n_walks, int variant, uint32_t* counters, int threads) {
+  if (variant != 0 && variant != 1) return -1;
+  if (variant == 1 && n_walks % 8 != 0) return -1;
+  if (src_end > n) src_end = static_cast<uint32_t>(n);
+  const int nt = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel num_threads(nt)
+  {
+    std::vector<uint32_t> local(n, 0u);
+    std::mt19937 rng(42u);
+#pragma omp for schedule(dynamic, 1024)
+    for (int64_t s = src_begin; s < static_cast<int64_t>(src_end); ++s) {
+      const uint32_t src = static_cast<uint32_t>(s);
+      if (deg(idx, src) == 0) continue;
+      rng.seed(42u + src);
+      if (variant == 0) {
+        for (uint64_t i = 0; i < n_walks; ++i) {
+          uint32_t cur = src;
+          for (uint32_t h = 0; h < depth; ++h) {
+            const uint32_t k = rng() % deg(idx, cur);
+            cur = adj[idx[cur] + k];
+            ++local[cur];
+          }
+        }
+      } else {
+        for (uint64_t ii = 0; ii < n_walks; ii += 8) {
+          uint32_t cur[8];
+          for (int j = 0; j < 8; ++j) cur[j] = src;
+          for (uint32_t h = 0; h < depth; ++h) {
+            uint32_t rnd[8];
+            for (int j = 0; j < 8; ++j) rnd[j] = rng();
+            for (int j = 0; j < 8; ++j) cur[j] = adj[idx[cur[j]] + rnd[j] % deg(idx, cur[j])];
+            for (int j = 0; j < 8; ++j) ++local[cur[j]];
+          }
+        }
+      }
+    }
+#pragma omp critical
+    for (uint64_t v = 0; v < n; ++v) counters[v] += local[v];
+  }
+  return 0;
+}

@@ -21,6 +21,12 @@ def load_golden(name: str) -> dict:
 
 
 GOLDEN_CASES = sorted(p.stem[len("ref_mt_"):] for p in GOLDEN.glob("ref_mt_*.npz"))
+MT_WALK_CASES = sorted(p.stem[len("ref_rwmt_"):] for p in GOLDEN.glob("ref_rwmt_*.npz"))
+
+
+def load_mt_walk_golden(name: str) -> dict:
+    with np.load(GOLDEN / f"ref_rwmt_{name}.npz") as z:
+        return {k: z[k] for k in z.files}
 
 
 @pytest.fixture(scope="session")

@@ -6,13 +6,17 @@ Sources of truth:
     (src/rw_experiment_saba.cpp:65-145) recorded when it was compiled and run
     in the survey container: C9 (40 hops, 64 walks) and K5 (20 hops, 64
     walks), both forcing the whole-bouquet SplitMix32 fallback.
+  * ref_rwmt_<case>.npz: per-vertex visit counts of the reference's own
+    scalar_randomwalk (src/rw_experiment.cpp:34-47, per-source std::mt19937
+    reseeded 42 + source) through oracle/_ref/ref_mt_walk, for several
+    (depth, n_walks) per graph.
   * ref_mt_<case>.npz: outputs of oracle/_ref/ref_tgtp_mt, i.e. the
     reference's own graph.h loader, EigenVecs::load_txt, calculate_taus and
     AESC::tgtp() of src/aesc_baseline.h (scalar MT19937 walks), compiled from
     /root/reference by `make -C oracle ref`, run on graphs from this
     package's seeded generators with eigenpairs from tgtp_amd.eigenprep.
 
-usage: python tests/golden/make_golden.py
+usage: python tests/golden/make_golden.py [--only-mt-walks]
 """
 from __future__ import annotations
 
@@ -32,6 +36,39 @@ from tgtp_amd import graph as G  # noqa: E402
 
 HERE = Path(__file__).resolve().parent
 REF = ROOT / "oracle" / "_ref" / "ref_tgtp_mt"
+REF_WALK = ROOT / "oracle" / "_ref" / "ref_mt_walk"
+
+# text of a graph with duplicates, a self-loop and an isolated vertex (4)
+ODD_GRAPH = "0 1\n0 1\n1 2\n2 2\n2 3\n5 3\n5 0\n6 5\n"
+
+# (name, graph factory, [(depth, n_walks), ...])
+MT_WALK_CASES = [
+    ("cycle15", lambda: G.cycle(15), [(5, 3), (5, 64)]),
+    ("petersen", lambda: G.petersen(), [(15, 8), (1, 40)]),
+    ("odd7", None, [(6, 16), (3, 5)]),
+    ("grid24", lambda: G.grid_rewired(24, 24, 0.05, seed=2), [(100, 20), (2, 200)]),
+    ("chunglu2000", lambda: G.chung_lu(2000, 2.5, 21.3, seed=5), [(5, 64), (10, 24)]),
+    ("chunglu4096", lambda: G.chung_lu(4096, 2.5, 22.0, seed=1), [(5, 2048)]),
+]
+
+
+def mt_walk_fixtures(tmp: str) -> None:
+    for name, make, runs in MT_WALK_CASES:
+        gtxt = f"{tmp}/{name}.graph.txt"
+        if make is None:
+            Path(gtxt).write_text(ODD_GRAPH)
+        else:
+            make().write_txt(gtxt)
+        g = G.load_txt(gtxt)
+        out = {"idx": g.idx, "adj": g.adj, "runs": np.array(runs, dtype=np.int64)}
+        for depth, n_walks in runs:
+            f = f"{tmp}/{name}.d{depth}.w{n_walks}.f32"
+            subprocess.run([str(REF_WALK), gtxt, str(depth), str(n_walks), f], check=True)
+            c = np.fromfile(f, np.float32)
+            assert c.max() < 2 ** 24 and np.array_equal(c, np.floor(c))
+            out[f"counts_d{depth}_w{n_walks}"] = c.astype(np.uint32)
+        np.savez_compressed(HERE / f"ref_rwmt_{name}.npz", **out)
+        print("mt walks", name, "n", g.n, "m", g.m, runs)
 
 # (name, graph factory, omega, epsilons, gamma)
 CASES = [
@@ -55,8 +92,13 @@ SABA_KATS = {
 
 
 def main() -> None:
-    if not REF.exists():
-        raise SystemExit(f"{REF} missing: run `make -C oracle ref` where /root/reference exists")
+    if not REF.exists() or not REF_WALK.exists():
+        raise SystemExit(f"{REF} / {REF_WALK} missing: run `make -C oracle ref` where "
+                         "/root/reference exists")
+    with tempfile.TemporaryDirectory() as tmp:
+        mt_walk_fixtures(tmp)
+    if "--only-mt-walks" in sys.argv:
+        return
     (HERE / "saba_kats.json").write_text(json.dumps(SABA_KATS, indent=2) + "\n")
     with tempfile.TemporaryDirectory() as tmp:
         for name, make, omega, epsilons, gamma in CASES:

@@ -109,3 +109,32 @@ def test_rw_cli_tsv_row(tmp_path):
     res = (tmp_path / "results" / "rw" / "saba_cl1000_d5_w64.txt").read_text().splitlines()
     kv = dict(line.split(" ", 1) for line in res)
     assert int(kv["visit_count_exact"]) == int(cnt.sum())
+
+
+def test_mt_rw_cli_validation():
+    r = run([BIN / "rw_experiment_simd", "d", "g", "5", "12"])
+    assert r.returncode == 1 and "n_walks must be a multiple of 8" in r.stderr
+    for b in ("rw_experiment", "rw_experiment_simd"):
+        assert "Usage" in run([BIN / b]).stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("binary,variant,label", [("rw_experiment", 0, "regular"),
+                                                  ("rw_experiment_simd", 1, "simd")])
+def test_mt_rw_cli_tsv_row(tmp_path, binary, variant, label):
+    g = G.chung_lu(1000, 2.5, 12.0, seed=14)
+    d = tmp_path / "data" / "cl1000"
+    d.mkdir(parents=True)
+    g.write_txt(d / "graph.txt")
+    r = run([BIN / binary, tmp_path / "data", "cl1000", "5", "64", "2"], cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    cols = r.stdout.rstrip("\n").split("\t")
+    assert len(cols) == 14 and cols[0] == label and cols[1] == "cl1000"
+    assert cols[2:7] == ["5", "64", "2", str(g.n), str(g.m)]
+    cnt = oracle.mt_walks(g.idx, g.adj, 5, 64, variant=variant).astype(np.uint64)
+    vc = float(cnt.sum())
+    cs = float((np.arange(1, g.n + 1, dtype=np.uint64) * cnt).sum())
+    assert cols[12] == f"{vc:g}" and cols[13] == f"{cs:g}"
+    res = (tmp_path / "results" / "rw" / f"{label}_cl1000_d5_w64.txt").read_text().splitlines()
+    kv = dict(line.split(" ", 1) for line in res)
+    assert int(kv["checksum_exact"]) == int((np.arange(1, g.n + 1, dtype=np.uint64) * cnt).sum())

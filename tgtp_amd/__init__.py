@@ -8,7 +8,8 @@ Layers:
   tgtp_amd.aesc                                  the reference's AESC interface
   tgtp_amd/bin/{tgtp,benchmark,rw_experiment_saba}  CLI drop-ins
 """
-from ._lib import (MODE_FAITHFUL, MODE_FAST_BF16, MODE_FAST_FP32, TgtpError)  # noqa: F401
+from ._lib import (MODE_FAITHFUL, MODE_FAST_BF16, MODE_FAST_FP32, WALK_MT_REGULAR,  # noqa: F401
+                   WALK_MT_SIMD, TgtpError)
 from .graph import (Graph, chung_lu, complete, cycle, grid_rewired, load_eigens_txt,  # noqa: F401
                     load_txt, petersen)
 from .runtime import Context, DeviceGraph  # noqa: F401

@@ -37,10 +37,13 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-std=c++20", "-O2", "-fPIC", "-fopenmp", "-ffp-contract=off",
               f"-I{INCLUDE}", f"-I{CSRC}"]
 
-HIP_SOURCES = ["capi.hip", "saba_walk.hip", "tgtp.hip", "tgtp_fast.hip", "fast_block.hip", "taus_fast.hip", "tgtp_batch.hip"]
+HIP_SOURCES = ["capi.hip", "saba_walk.hip", "mt_walk.hip", "tgtp.hip", "tgtp_fast.hip", "fast_block.hip", "taus_fast.hip", "tgtp_batch.hip"]
 HOST_SOURCES = ["graph_io.cpp", "host_tgtp.cpp"]
-CLI_SOURCES = {"rw_experiment_saba": "cli_rw_saba.cpp", "tgtp": "cli_tgtp.cpp",
-               "benchmark": "cli_benchmark.cpp"}
+# binary -> (source, extra defines)
+CLI_SOURCES = {"rw_experiment_saba": ("cli_rw.cpp", []),
+               "rw_experiment": ("cli_rw.cpp", ["-DTGTP_RW_VARIANT=1"]),
+               "rw_experiment_simd": ("cli_rw.cpp", ["-DTGTP_RW_VARIANT=2"]),
+               "tgtp": ("cli_tgtp.cpp", []), "benchmark": ("cli_benchmark.cpp", [])}
 
 
 def _run(cmd: list[str]) -> None:
@@ -79,10 +82,10 @@ def build(force: bool = True, verbose: bool = False) -> Path:
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *objs, "-lgomp"])
     os.replace(tmp, LIB)
     clis = []
-    for name, src in CLI_SOURCES.items():
+    for name, (src, defines) in CLI_SOURCES.items():
         if not (CSRC / src).exists():
             continue
-        clis.append(["g++", "-std=c++20", "-O2", f"-I{INCLUDE}", str(CSRC / src), "-o",
+        clis.append(["g++", "-std=c++20", "-O2", *defines, f"-I{INCLUDE}", str(CSRC / src), "-o",
                      str(BIN / name), f"-L{PKG}", "-ltgtp_hip", f"-Wl,-rpath,{PKG}",
                      "-Wl,-rpath,$ORIGIN/..", "-fopenmp"])
     with cf.ThreadPoolExecutor(max_workers=4) as ex:

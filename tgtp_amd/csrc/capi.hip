@@ -174,6 +174,50 @@ extern "C" void tgtp_graph_free(tgtp_ctx* ctx, tgtp_dgraph* dg) {
   delete dg;
 }
 
+extern "C" int tgtp_mt_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,
+                             uint32_t src_end, uint32_t depth, uint32_t n_walks, int32_t variant,
+                             uint32_t* counters_out, uint64_t* visit_count, uint64_t* checksum,
+                             tgtp_stats* stats) {
+  if (!ctx) return TGTP_E_ARG;
+  if (!dg) return set_error(ctx, TGTP_E_ARG, "null device graph");
+  if (variant != TGTP_WALK_MT_REGULAR && variant != TGTP_WALK_MT_SIMD)
+    return set_error(ctx, TGTP_E_ARG, "unknown walk variant");
+  if (variant == TGTP_WALK_MT_SIMD && n_walks % 8 != 0)
+    return set_error(ctx, TGTP_E_ARG, "n_walks must be a multiple of 8");
+  if (src_end > dg->n || src_begin > src_end)
+    return set_error(ctx, TGTP_E_ARG, "source range outside [0, n]");
+  const double t0 = now_ms();
+  TGTP_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  uint32_t* d_counters =
+      static_cast<uint32_t*>(ctx_buffer(ctx, kBufCounters, dg->n * sizeof(uint32_t)));
+  auto* d_sums = static_cast<unsigned long long*>(ctx_buffer(ctx, kBufSums, 64));
+  if (!d_counters || !d_sums) return set_error(ctx, TGTP_E_NOMEM, "counter buffers");
+  float walk_ms = 0;
+  int rc = launch_mt_walks(ctx, dg, src_begin, src_end, depth, n_walks, variant, d_counters,
+                           d_sums, &walk_ms);
+  if (rc) return rc;
+  unsigned long long sums[2] = {0, 0};
+  TGTP_HIP_CHECK(ctx, hipMemcpyAsync(sums, d_sums, sizeof(sums), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+  if (counters_out)
+    TGTP_HIP_CHECK(ctx, hipMemcpyAsync(counters_out, d_counters, dg->n * sizeof(uint32_t),
+                                       hipMemcpyDeviceToHost, ctx->stream));
+  TGTP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (visit_count) *visit_count = sums[0];
+  if (checksum) *checksum = sums[1];
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    float all_ms = 0;
+    TGTP_HIP_CHECK(ctx, hipEventElapsedTime(&all_ms, ctx->ev[0], ctx->ev[3]));
+    stats->kernel_ms = all_ms;
+    stats->walk_ms = walk_ms;
+    stats->transitions = sums[0];
+    stats->work_items = static_cast<uint64_t>(src_end - src_begin) * n_walks;
+    stats->e2e_ms = now_ms() - t0;
+  }
+  return TGTP_OK;
+}
+
 extern "C" int tgtp_saba_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,
                                uint32_t src_end, uint32_t depth, uint32_t n_walks,
                                uint32_t* counters_out, uint64_t* visit_count,

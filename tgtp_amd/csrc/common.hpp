@@ -107,6 +107,16 @@ int set_error(tgtp_ctx* ctx, int code, const std::string& msg);
   } while (0)
 
 // ---- kernel launchers (defined in the .hip files) ----------------------------
+// Walk counters (saba_walk.hip): clear counters [n] and sums [2]; then
+// sums[0] += sum(c), sums[1] += sum((v+1) c[v]).
+int launch_walk_counters_clear(tgtp_ctx* ctx, uint32_t* d_counters, uint64_t n,
+                               unsigned long long* d_sums);
+int launch_walk_counter_sums(tgtp_ctx* ctx, const uint32_t* d_counters, uint64_t n,
+                             unsigned long long* d_sums);
+// MT19937 comparator walks (mt_walk.hip); variant 0 regular, 1 simd.
+int launch_mt_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin, uint32_t src_end,
+                    uint32_t depth, uint32_t n_walks, int32_t variant, uint32_t* d_counters,
+                    unsigned long long* d_sums, float* walk_ms);
 int launch_saba_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,
                       uint32_t src_end, uint32_t depth, uint32_t n_walks,
                       uint32_t* d_counters, unsigned long long* d_sums,

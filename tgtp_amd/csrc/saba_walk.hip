@@ -420,6 +420,23 @@ std::vector<double> bucket_mass(tgtp_ctx* ctx, tgtp_dgraph* dg, uint32_t hot, ui
 
 }  // namespace
 
+int launch_walk_counters_clear(tgtp_ctx* ctx, uint32_t* d_counters, uint64_t n,
+                               unsigned long long* d_sums) {
+  fill_u32_kernel<<<std::min<uint64_t>(4096, (n + 255) / 256 + 1), 256, 0, ctx->stream>>>(
+      d_counters, n, 0u);
+  fill_u64_kernel<<<1, 64, 0, ctx->stream>>>(d_sums, 2);
+  TGTP_HIP_CHECK(ctx, hipGetLastError());
+  return TGTP_OK;
+}
+
+int launch_walk_counter_sums(tgtp_ctx* ctx, const uint32_t* d_counters, uint64_t n,
+                             unsigned long long* d_sums) {
+  counter_sums_kernel<<<std::min<uint64_t>(1024, (n + 255) / 256 + 1), 256, 0, ctx->stream>>>(
+      d_counters, n, d_sums);
+  TGTP_HIP_CHECK(ctx, hipGetLastError());
+  return TGTP_OK;
+}
+
 int launch_saba_walks(tgtp_ctx* ctx, const tgtp_dgraph* cdg, uint32_t src_begin,
                       uint32_t src_end, uint32_t depth, uint32_t n_walks,
                       uint32_t* d_counters, unsigned long long* d_sums, float* walk_ms) {
@@ -430,9 +447,7 @@ int launch_saba_walks(tgtp_ctx* ctx, const tgtp_dgraph* cdg, uint32_t src_begin,
   int rc = ensure_walk_adj(ctx, dg, &packed);
   if (rc) return rc;
   TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], s));
-  fill_u32_kernel<<<std::min<uint64_t>(4096, (n + 255) / 256 + 1), 256, 0, s>>>(d_counters, n, 0u);
-  fill_u64_kernel<<<1, 64, 0, s>>>(d_sums, 2);
-  TGTP_HIP_CHECK(ctx, hipGetLastError());
+  if ((rc = launch_walk_counters_clear(ctx, d_counters, n, d_sums))) return rc;
   TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], s));
 
   const uint64_t n_src = src_end > src_begin ? src_end - src_begin : 0;
@@ -597,9 +612,7 @@ int launch_saba_walks(tgtp_ctx* ctx, const tgtp_dgraph* cdg, uint32_t src_begin,
     }
   }
   TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2], s));
-  counter_sums_kernel<<<std::min<uint64_t>(1024, (n + 255) / 256 + 1), 256, 0, s>>>(d_counters, n,
-                                                                                  d_sums);
-  TGTP_HIP_CHECK(ctx, hipGetLastError());
+  if ((rc = launch_walk_counter_sums(ctx, d_counters, n, d_sums))) return rc;
   TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[3], s));
   TGTP_HIP_CHECK(ctx, hipEventSynchronize(ctx->ev[3]));
   if (walk_ms) TGTP_HIP_CHECK(ctx, hipEventElapsedTime(walk_ms, ctx->ev[1], ctx->ev[2]));

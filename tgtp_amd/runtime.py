@@ -142,6 +142,20 @@ class Context:
         _check(self.ptr, rc, "tgtp_saba_walks")
         return cnt, int(vc.value), int(cs.value), st.as_dict()
 
+    def mt_walks(self, dg: DeviceGraph, depth: int, n_walks: int, variant: int = L.WALK_MT_REGULAR,
+                 src_begin: int = 0, src_end: int | None = None, want_counters: bool = True):
+        """MT19937 comparator walks (tgtp_mt_walks) -> (counters uint32[n] or
+        None, visit_count, checksum, stats dict)."""
+        src_end = dg.n if src_end is None else src_end
+        cnt = np.zeros(dg.n, dtype=np.uint32) if want_counters else None
+        vc, cs = C.c_uint64(), C.c_uint64()
+        st = L.StatsStruct()
+        rc = L.lib().tgtp_mt_walks(self.ptr, dg.ptr, src_begin, src_end, depth, n_walks, variant,
+                                   L.ptr(cnt, C.c_uint32) if cnt is not None else None,
+                                   C.byref(vc), C.byref(cs), C.byref(st))
+        _check(self.ptr, rc, "tgtp_mt_walks")
+        return cnt, int(vc.value), int(cs.value), st.as_dict()
+
     @staticmethod
     def _eigens(lam: np.ndarray, X: np.ndarray):
         lam = np.ascontiguousarray(lam, dtype=np.float64)
