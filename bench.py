@@ -10,9 +10,14 @@ reference's t_rw_s (src/rw_experiment_saba.cpp:184-211).  The graph is
 resident in HBM before timing starts.
 
 value = transitions/s = n * n_walks * depth / seconds per step
-(rw_experiment_saba.cpp:214-215).  One JSON line on stdout.
+(rw_experiment_saba.cpp:214-215).  One JSON line on stdout.  Further
+objects in the line: the host CPU baseline of the walks, `mt_walks` (the
+MT19937 comparator walks), `tgtp` (n=4096 Chung-Lu, FAITHFUL and FAST),
+`tgtp_batch` (256 graphs, batched vs plain) and, with --precision, the
+512 x 512 rewired grid.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-tgtp]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-mt]
+                    [--no-tgtp] [--no-batch] [--precision]
 """
 from __future__ import annotations
 
@@ -407,6 +412,59 @@ def cpu_saba_baseline(g, depth, n_walks, sample_sources: int):
                       f"oracle C++ OpenMP dynamic chunk 8, uint32 counters"}
 
 
+def bench_mt(args, g_big=None):
+    """MT19937 comparator walks (rw_experiment.cpp / rw_experiment_simd.cpp)
+    on the GPU beside the oracle on the host: the n=4096 walk config in full
+    (5 hops, 2048 walks, counters compared bit for bit) and, when given, 16
+    evenly spread slices of 4096 sources of the n=2^20 graph (10 hops, 16384
+    walks)."""
+    import numpy as np
+
+    import oracle
+    from tgtp_amd import Context, WALK_MT_REGULAR, WALK_MT_SIMD, workloads as WL
+
+    threads = host_threads()
+    ctx = Context(0)
+    out = {}
+    g = WL.chung_lu_4096()
+    dg = ctx.upload(g)
+    leg = {"workload": f"chung-lu n=4096 m={g.m} depth=5 n_walks=2048, all sources"}
+    for name, v in (("regular", WALK_MT_REGULAR), ("simd", WALK_MT_SIMD)):
+        ctx.mt_walks(dg, 5, 2048, v)
+        best, cnt, vc = 1e30, None, 0
+        for _ in range(3):
+            cnt, vc, cs, st = ctx.mt_walks(dg, 5, 2048, v)
+            best = min(best, st["kernel_ms"])
+        t0 = time.perf_counter()
+        ref = oracle.mt_walks(g.idx, g.adj, 5, 2048, variant=v, threads=threads)
+        tc = time.perf_counter() - t0
+        leg[name] = {"gpu_transitions_per_s": vc / (best / 1e3), "gpu_ms": round(best, 3),
+                     "cpu_transitions_per_s": vc / tc, "cpu_cores": threads,
+                     "bitwise_equal_oracle": bool(np.array_equal(cnt, ref))}
+        log(f"[bench] mt {name} n=4096: gpu {vc / (best / 1e3):.3e} transitions/s, "
+            f"cpu ({threads} threads) {vc / tc:.3e}, equal={leg[name]['bitwise_equal_oracle']}")
+    out["cl4096_d5_w2048"] = leg
+    dg.free()
+    if g_big is not None:
+        dg = ctx.upload(g_big)
+        step, per = g_big.n // 16, 4096
+        leg = {"workload": f"chung-lu n=2^20 m={g_big.m} depth=10 n_walks=16384, "
+                           f"16 slices x {per} sources"}
+        for name, v in (("regular", WALK_MT_REGULAR), ("simd", WALK_MT_SIMD)):
+            ms, total = 0.0, 0
+            for k in range(16):
+                _, vc, _, st = ctx.mt_walks(dg, 10, 16384, v, k * step, k * step + per,
+                                           want_counters=False)
+                ms += st["walk_ms"]
+                total += vc
+            leg[name] = {"gpu_transitions_per_s": total / (ms / 1e3), "gpu_walk_ms": round(ms, 1)}
+            log(f"[bench] mt {name} n=2^20 sample: gpu {total / (ms / 1e3):.3e} transitions/s")
+        out["cl1m_d10_w16384_sample"] = leg
+        dg.free()
+    ctx.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -419,6 +477,7 @@ def main() -> int:
     ap.add_argument("--no-tgtp", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-saba", action="store_true", help="tgtp legs only (profiling)")
+    ap.add_argument("--no-mt", action="store_true", help="skip the MT19937 comparator walks")
     ap.add_argument("--precision", action="store_true",
                     help="add the 512x512 rewired-grid precision leg (minutes)")
     ap.add_argument("--precision-side", type=int, default=512)
@@ -454,6 +513,8 @@ def main() -> int:
         line["cpu_baseline"] = cpu_saba_baseline(g, args.depth, args.walks,
                                                  args.cpu_sample_sources)
         line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+    if not args.no_mt:
+        line["mt_walks"] = bench_mt(args, g)
     del g
     if not args.no_tgtp:
         line["tgtp"] = bench_tgtp(args)
