@@ -362,13 +362,14 @@ blk_round4_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_units,
     const uint32_t base = base0 + 16u * g;
     if (base >= rend) break;  // uniform
     const uint32_t u = base + qd;
-    const bool rv = u < rend;
     const uint2 r = rr[g];
+    // rows above the block's hub degree belong to blk_hub_kernel
+    const bool rv = u < rend && (B.hub_deg == 0 || r.y <= B.hub_deg);
     const uint32_t my0 = my_next;
     if (g + 1 < kGroups) my_next = row_id(ars, rr[g + 1], voff, ql, 0);
     // the row's degree is the same in all 16 lanes of the quarter, whatever
     // their sources' flags: every lane fetches its share of the ids
-    const uint32_t deg = r.y;
+    const uint32_t deg = rv ? r.y : 0u;
     const uint32_t dmax = max(max(__builtin_amdgcn_readlane(deg, 0), __builtin_amdgcn_readlane(deg, 16)),
                               max(__builtin_amdgcn_readlane(deg, 32), __builtin_amdgcn_readlane(deg, 48)));
     const uint64_t ro = static_cast<uint64_t>(u) * kBlkCols;
@@ -450,6 +451,80 @@ blk_round4_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_units,
       atomicAdd(&A.st_q[col], fq);
       atomicAdd(&A.st_dsum[col], fd);
     }
+  }
+}
+
+// Last block in [b0, b1) whose first hub row is <= t.
+__device__ __forceinline__ uint32_t find_hub_block(const BlockInfo* B, uint32_t b0, uint32_t b1,
+                                                   uint32_t t) {
+  uint32_t lo = b0, hi = b1;
+  while (hi - lo > 1) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (B[mid].hub0 <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// One round of one hub row (degree above the block's hub_deg) of one block per
+// workgroup: the row's neighbours in 16 contiguous segments, one per
+// quarter-wave (lane ql holding sources 4 ql .. 4 ql + 3), each summed in
+// adjacency order; the 16 segment sums are then added in segment order.  A
+// long row thus costs ~deg / 256 dependent gather steps instead of deg / 16
+// in one quarter of blk_round4_kernel, which skips these rows.
+__global__ void __launch_bounds__(256)
+blk_hub_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_units,
+               const float* __restrict__ cur, float* __restrict__ nxt) {
+  constexpr uint32_t kSeg = 16;
+  __shared__ float4 part[kSeg][16];
+  const uint32_t unit = blockIdx.x;
+  if (unit >= n_units) return;
+  const uint32_t b = find_hub_block(A.blocks, b0, b1, unit);
+  const BlockInfo B = A.blocks[b];
+  const uint32_t voff = B.voff;
+  const uint32_t v = A.hubs[A.hub_off[B.gi] + (unit - B.hub0)] - voff;
+  const uint32_t l = lane_id(), wv = wave_id();
+  const uint32_t qd = l >> 4, ql = l & 15u;
+  const uchar4 a4 = *reinterpret_cast<const uchar4*>(A.active + b * kBlkCols + ql * 4u);
+  const bool any = a4.x || a4.y || a4.z || a4.w;
+  if (__ballot(any) == 0ull) return;  // same flags in every wave
+  const uint2 r = A.rowd[voff + v];
+  const uint32_t len = (r.y + kSeg - 1) / kSeg;
+  const uint32_t seg = wv * 4u + qd;
+  const uint32_t k0 = min(r.y, seg * len), k1 = min(r.y, k0 + len);
+  const __amdgpu_buffer_rsrc_t xrs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float*>(cur + B.xoff), 0, static_cast<int>(B.n * (kBlkCols * 4u)), kRsrcWord3);
+  const __amdgpu_buffer_rsrc_t ars = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<uint32_t*>(A.adj), 0, static_cast<int>(A.adj_words * 4u), kRsrcWord3);
+  const uint2 seg_row = make_uint2(r.x, k1);  // ids past the segment read out of range
+  float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  uint32_t my = row_id(ars, seg_row, voff, ql, k0);
+  for (uint32_t k = k0; k < k1; k += 16) {
+    const uint32_t my_n = row_id(ars, seg_row, voff, ql, k + 16);
+    gather_ids<16>(xrs, my, k1, any, l, k, sum);
+    my = my_n;
+  }
+  part[seg][ql] = sum;
+  __syncthreads();
+  if (wv != 0) return;
+  const uint32_t j = l, col = b * kBlkCols + j;
+  float t = 0.0f;
+#pragma unroll
+  for (uint32_t q = 0; q < kSeg; ++q) {
+    const float4 p4 = part[q][j >> 2];
+    t += (j & 3u) == 0 ? p4.x : ((j & 3u) == 1 ? p4.y : ((j & 3u) == 2 ? p4.z : p4.w));
+  }
+  const uint64_t ro = static_cast<uint64_t>(v) * kBlkCols + j;
+  const bool act = A.active[col] != 0;
+  const float val = act ? t / static_cast<float>(r.y) : cur[B.xoff + ro];
+  nxt[B.xoff + ro] = val;
+  if (act && val > 0.0f) {
+    atomicMax(&A.st_mx[col], __float_as_uint(val));
+    atomicMin(&A.st_mn[col], __float_as_uint(val));
+    atomicAdd(&A.st_q[col], 1u);
+    atomicAdd(&A.st_dsum[col], static_cast<unsigned long long>(r.y));
   }
 }
 
@@ -955,8 +1030,8 @@ int block_post_per_cu(bool staged) {
 }
 
 int launch_block_subset(tgtp_ctx* ctx, hipStream_t s, const BlockArgs& A, const BlockSubset& S,
-                        const uint32_t* round_tiles, uint32_t post_slots,
-                        uint32_t post_slots_plain) {
+                        const uint32_t* round_tiles, const uint32_t* round_hubs,
+                        uint32_t post_slots, uint32_t post_slots_plain) {
   const uint32_t nb = S.b1 - S.b0;
   const uint32_t col_blocks = (nb * kBlkCols + 3) / 4;
   const uint64_t zb = std::min<uint64_t>(16384, (S.floats + 255) / 256 + 1);
@@ -979,6 +1054,8 @@ int launch_block_subset(tgtp_ctx* ctx, hipStream_t s, const BlockArgs& A, const 
       blk_round4_kernel<16, 4><<<grid_for(tr), 256, 0, s>>>(A, S.b0, S.b1, tr, cur, nxt);
     else
       blk_round_kernel<<<grid_for(tr), 256, 0, s>>>(A, S.b0, S.b1, tr, cur, nxt);
+    if (A.round_form != 0 && round_hubs[r] != 0)
+      blk_hub_kernel<<<round_hubs[r], 256, 0, s>>>(A, S.b0, S.b1, round_hubs[r], cur, nxt);
     blk_stop_kernel<<<col_blocks, 256, 0, s>>>(A, S.b0, S.b1, nxt);
   }
   blk_transpose_kernel<<<tile_grid, 256, 0, s>>>(A, S.b0, S.b1, S.tiles);

@@ -32,6 +32,8 @@ struct BlockInfo {
   uint32_t tile0;  // tile units of the subset before this block: tiles
                    // (tile_mode 0) or groups of 8 consecutive tiles (mode 1)
   uint64_t xoff;   // float offset of the block's n x 64 state in the subset buffers
+  uint32_t hub0;     // hub rows of the subset's blocks before this block
+  uint32_t hub_deg;  // rows above this degree are the graph's hub rows (0: none)
 };
 
 struct BlockArgs {
@@ -65,6 +67,10 @@ struct BlockArgs {
   uint32_t small_bq;   // walk positions with at most this many bouquets: one 8-lane group
   uint32_t round_form; // 0: one row per wave; 1 / 2: four rows per wave, 8 / 16 gathers per step
   uint32_t adj_words;  // CSR entries (the four-rows form needs m < 2^30)
+  // rows of degree above a block's hub_deg are summed by blk_hub_kernel;
+  // hubs[hub_off[g] ..] lists graph g's, as union vertex ids
+  const uint32_t* hubs;
+  const uint32_t* hub_off;
   int32_t* tr_ell;
   float* tr_emax;
   float* tr_gmax;
@@ -77,7 +83,7 @@ struct BlockSubset {
   uint32_t b0, b1;     // blocks [b0, b1)
   uint32_t tiles;      // tile units (see BlockInfo::tile0)
   uint32_t rounds;     // rounds to launch (>= every source's stopping round)
-  uint32_t round_tiles0;  // index of this subset's per-round tile counts
+  uint32_t round_tiles0;  // index of this subset's per-round tile and hub counts
   uint64_t floats;     // state floats per buffer
   uint32_t n_min;      // vertices of the subset's smallest graph
 };
@@ -85,10 +91,12 @@ struct BlockSubset {
 // Enqueues the whole per-source phase for one subset on `stream`.
 // round_tiles[r]: the tile units round r covers (the subset's blocks are
 // ordered by decreasing round bound, so they are a prefix).
+// round_hubs[r]: the hub rows round r covers (a prefix, as the tiles).
 // post_slots / post_slots_plain: post kernel workgroups for subsets with /
 // without vectors staged in LDS.
 int launch_block_subset(tgtp_ctx* ctx, hipStream_t stream, const BlockArgs& A,
-                        const BlockSubset& S, const uint32_t* round_tiles, uint32_t post_slots,
+                        const BlockSubset& S, const uint32_t* round_tiles,
+                        const uint32_t* round_hubs, uint32_t post_slots,
                         uint32_t post_slots_plain);
 // Resident workgroups per CU of the post kernel, launched with (staged) or
 // without its LDS vector buffer.
