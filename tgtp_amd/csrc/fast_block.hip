@@ -979,8 +979,9 @@ __device__ void post_source(const BlockArgs& A, const BlockInfo& B, uint32_t col
   __syncthreads();
 }
 
-// Vectors of blocks with n <= stage_max (0 or kPostLds, the dynamic LDS the
-// launch provides) are staged in LDS; larger ones are read in place.
+// Vectors of blocks with n <= stage_max (the floats of dynamic LDS the
+// launch provides, at most kPostLds) are staged in LDS; larger ones are
+// read in place.
 __global__ void __launch_bounds__(kPB) blk_post_kernel(BlockArgs A, uint32_t b0, uint32_t b1,
                                                        uint32_t stage_max) {
   extern __shared__ float lds_pv[];
@@ -1020,18 +1021,18 @@ __global__ void __launch_bounds__(kPB) blk_post_kernel(BlockArgs A, uint32_t b0,
 
 }  // namespace
 
-int block_post_per_cu(bool staged) {
+int block_post_per_cu(uint32_t stage) {
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
           &per_cu, reinterpret_cast<const void*>(&blk_post_kernel), kPB,
-          staged ? kPostLds * sizeof(float) : 0) != hipSuccess)
+          static_cast<size_t>(stage) * sizeof(float)) != hipSuccess)
     per_cu = 1;
   return std::max(1, per_cu);
 }
 
 int launch_block_subset(tgtp_ctx* ctx, hipStream_t s, const BlockArgs& A, const BlockSubset& S,
                         const uint32_t* round_tiles, const uint32_t* round_hubs,
-                        uint32_t post_slots, uint32_t post_slots_plain) {
+                        uint32_t post_slots) {
   const uint32_t nb = S.b1 - S.b0;
   const uint32_t col_blocks = (nb * kBlkCols + 3) / 4;
   const uint64_t zb = std::min<uint64_t>(16384, (S.floats + 255) / 256 + 1);
@@ -1060,13 +1061,11 @@ int launch_block_subset(tgtp_ctx* ctx, hipStream_t s, const BlockArgs& A, const 
   }
   blk_transpose_kernel<<<tile_grid, 256, 0, s>>>(A, S.b0, S.b1, S.tiles);
   blk_zero_kernel<<<1, 256, 0, s>>>(reinterpret_cast<float*>(A.next_src), 1);
-  // a subset without vectors to stage runs without the LDS buffer, so more
-  // workgroups share a CU
-  const bool staged = S.n_min <= kPostLds;
-  const uint32_t slots = staged ? post_slots : post_slots_plain;
-  blk_post_kernel<<<std::max(1u, std::min(slots, nb * kBlkCols)), kPB,
-                    staged ? kPostLds * sizeof(float) : 0, s>>>(A, S.b0, S.b1,
-                                                               staged ? kPostLds : 0u);
+  // the LDS buffer holds the subset's largest staged vector (smaller
+  // buffers, more workgroups per CU; none when every vector is read in place)
+  blk_post_kernel<<<std::max(1u, std::min(post_slots, nb * kBlkCols)), kPB,
+                    static_cast<size_t>(S.post_stage) * sizeof(float), s>>>(A, S.b0, S.b1,
+                                                                           S.post_stage);
   TGTP_HIP_CHECK(ctx, hipGetLastError());
   return TGTP_OK;
 }

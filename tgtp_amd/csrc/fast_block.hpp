@@ -85,19 +85,19 @@ struct BlockSubset {
   uint32_t rounds;     // rounds to launch (>= every source's stopping round)
   uint32_t round_tiles0;  // index of this subset's per-round tile and hub counts
   uint64_t floats;     // state floats per buffer
-  uint32_t n_min;      // vertices of the subset's smallest graph
+  uint32_t post_stage;   // floats of LDS per post workgroup: the largest vector
+                         // of the subset up to kPostLds (0: none staged)
+  uint32_t post_per_cu;  // post workgroups resident per CU with that LDS
 };
 
 // Enqueues the whole per-source phase for one subset on `stream`.
 // round_tiles[r]: the tile units round r covers (the subset's blocks are
 // ordered by decreasing round bound, so they are a prefix).
 // round_hubs[r]: the hub rows round r covers (a prefix, as the tiles).
-// post_slots / post_slots_plain: post kernel workgroups for subsets with /
-// without vectors staged in LDS.
+// post_slots: post kernel workgroups (resident ones; they loop over sources).
 int launch_block_subset(tgtp_ctx* ctx, hipStream_t stream, const BlockArgs& A,
                         const BlockSubset& S, const uint32_t* round_tiles,
-                        const uint32_t* round_hubs, uint32_t post_slots,
-                        uint32_t post_slots_plain);
-// Resident workgroups per CU of the post kernel, launched with (staged) or
-// without its LDS vector buffer.
-int block_post_per_cu(bool staged);
+                        const uint32_t* round_hubs, uint32_t post_slots);
+// Resident workgroups per CU of the post kernel launched with `stage`
+// floats of LDS vector buffer.
+int block_post_per_cu(uint32_t stage);

@@ -96,7 +96,7 @@ struct FastPlan {
   uint32_t* d_words = nullptr;
   float* d_out = nullptr;
   uint32_t max_slots = 0, slots = 0;  // resident workgroups of the post kernel
-  uint32_t per_cu_staged = 1, per_cu_plain = 1;  // ... per CU, with / without its LDS buffer
+  uint32_t per_cu_staged = 1;  // ... per CU with the full LDS buffer (what slots is tuned for)
   uint32_t tile_mode = 0;
   uint32_t round_form = 0;  // blk_round_kernel (0) or the four-rows-per-wave form (1, 2)
   // tuned parameters
@@ -411,8 +411,11 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
       rounds = std::max(rounds, br[i]);
     }
     S.b1 = b;
-    S.n_min = ~0u;
-    for (const BlockInfo& x : bs) S.n_min = std::min(S.n_min, x.n);
+    S.post_stage = 0;
+    for (const BlockInfo& x : bs)
+      if (x.n <= kPostLds) S.post_stage = std::max(S.post_stage, x.n);
+    S.post_stage = std::min(kPostLds, (S.post_stage + 255u) / 256u * 256u);
+    S.post_per_cu = static_cast<uint32_t>(block_post_per_cu(S.post_stage));
     S.tiles = tiles;
     S.rounds = rounds;
     S.floats = fl;
@@ -440,8 +443,7 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
       (rc = dev_alloc(ctx, &P->d_colu, 4 * cols + P->n_blocks, "column statistics")) ||
       (rc = dev_alloc(ctx, &P->d_dsum, cols, "column degree sums")))
     return fail(rc);
-  P->per_cu_staged = static_cast<uint32_t>(block_post_per_cu(true));
-  P->per_cu_plain = static_cast<uint32_t>(block_post_per_cu(false));
+  P->per_cu_staged = static_cast<uint32_t>(block_post_per_cu(kPostLds));
   P->max_slots = static_cast<uint32_t>(static_cast<uint64_t>(ctx->num_cus) * P->per_cu_staged);
   P->slots = P->max_slots;
   TGTP_HIP_CHECK(ctx, hipStreamSynchronize(s));
@@ -542,8 +544,8 @@ int enqueue_run(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_
     hipStream_t st = second ? P->side : ctx->stream;
     if ((rc = launch_block_subset(ctx, st, second ? A2 : A, S,
                                   P->round_tiles.data() + S.round_tiles0,
-                                  P->round_hubs.data() + S.round_tiles0, P->slots,
-                                  std::max(1u, P->slots * P->per_cu_plain / P->per_cu_staged))))
+                                  P->round_hubs.data() + S.round_tiles0,
+                                  std::max(1u, P->slots * S.post_per_cu / P->per_cu_staged))))
       return rc;
     if (progress) {
       TGTP_HIP_CHECK(ctx, hipStreamSynchronize(st));
