@@ -872,34 +872,63 @@ __device__ void post_source(const BlockArgs& A, const BlockInfo& B, uint32_t col
   smx = block_reduce<kPW>(smx, sh.rf, OpMaxF{});
   smn = block_reduce<kPW>(smn, sh.rf, OpMinF{});
   const float dse = static_cast<float>(dse_d);
-  for (uint32_t p = r0 + wave_id(); p < r1; p += kPW) {
-    const int rest = A.taus[p] - ell;
-    if (rest <= 0) {
-      if (lane_id() == 0) A.nr_pos[p] = 0;
-      continue;
+  auto emit_nr = [&](uint32_t p, int rest, float tmx, float tmn) {
+    const int len = rest;
+    const float chi = gmax + (smx + tmx) / 2.0 + (len - 1) * emax - smn - tmn -
+                      2 * (len - 1) * gmin;
+    const double v = fmax(ceil(8 * (chi * chi) * logmd / (dse * dse)), 1.0);
+    uint64_t nr = 0;
+    if (!(v < kMaxWalksFast)) {
+      atomicOr(A.status, 1u);
+    } else {
+      nr = static_cast<uint64_t>(v);
+      nr += 8 - nr % 8;
     }
-    const uint2 rv = A.rowd[A.adj[p]];
-    float tmx = 0.0f, tmn = FLT_MAX;
-    for (uint32_t k = lane_id(); k < rv.y; k += 64) {
-      const float x = pv[A.adj[rv.x + k] - off];
-      tmx = fmaxf(tmx, x);
-      tmn = fminf(tmn, x);
-    }
-    tmx = wave_max(tmx);
-    tmn = wave_min(tmn);
-    if (lane_id() == 0) {
-      const int len = rest;
-      const float chi = gmax + (smx + tmx) / 2.0 + (len - 1) * emax - smn - tmn -
-                        2 * (len - 1) * gmin;
-      const double v = fmax(ceil(8 * (chi * chi) * logmd / (dse * dse)), 1.0);
-      uint64_t nr = 0;
-      if (!(v < kMaxWalksFast)) {
-        atomicOr(A.status, 1u);
-      } else {
-        nr = static_cast<uint64_t>(v);
-        nr += 8 - nr % 8;
+    A.nr_pos[p] = nr;
+  };
+  if (rs.y > 64) {
+    // a hub source: one 8-lane group per position, 32 positions at a time
+    // (a wave per position made its positions the workgroup's longest
+    // chain); max and min are exact in any order
+    const uint32_t cg = threadIdx.x >> 3, cb = threadIdx.x & 7u;
+    for (uint32_t p = r0 + cg; p < r1; p += kPB / 8) {
+      const int rest = A.taus[p] - ell;
+      if (rest <= 0) {
+        if (cb == 0) A.nr_pos[p] = 0;
+        continue;
       }
-      A.nr_pos[p] = nr;
+      const uint2 rv = A.rowd[A.adj[p]];
+      float tmx = 0.0f, tmn = FLT_MAX;
+      for (uint32_t k = cb; k < rv.y; k += 8) {
+        const float x = pv[A.adj[rv.x + k] - off];
+        tmx = fmaxf(tmx, x);
+        tmn = fminf(tmn, x);
+      }
+#pragma unroll
+      for (int o = 4; o > 0; o >>= 1) {
+        tmx = fmaxf(tmx, __shfl_xor(tmx, o));
+        tmn = fminf(tmn, __shfl_xor(tmn, o));
+      }
+      if (cb == 0) emit_nr(p, rest, tmx, tmn);
+    }
+  } else {
+    // one wave per position: the neighbour's row in 64-wide steps
+    for (uint32_t p = r0 + wave_id(); p < r1; p += kPW) {
+      const int rest = A.taus[p] - ell;
+      if (rest <= 0) {
+        if (lane_id() == 0) A.nr_pos[p] = 0;
+        continue;
+      }
+      const uint2 rv = A.rowd[A.adj[p]];
+      float tmx = 0.0f, tmn = FLT_MAX;
+      for (uint32_t k = lane_id(); k < rv.y; k += 64) {
+        const float x = pv[A.adj[rv.x + k] - off];
+        tmx = fmaxf(tmx, x);
+        tmn = fminf(tmn, x);
+      }
+      tmx = wave_max(tmx);
+      tmn = wave_min(tmn);
+      if (lane_id() == 0) emit_nr(p, rest, tmx, tmn);
     }
   }
   __syncthreads();
