@@ -50,7 +50,7 @@ constexpr uint32_t kInfBits = 0x7f800000u;
 // tile0 of a block counts tiles (mode 0) or tile groups (mode 1) before it.
 __device__ __forceinline__ bool tile_of(const BlockInfo* blocks, uint32_t b0, uint32_t b1,
                                         uint32_t n_units, uint32_t mode, uint32_t* bo,
-                                        uint32_t* row0);
+                                        uint32_t* row0, uint32_t vb, uint32_t vgrid);
 
 // Last block in [b0, b1) whose first tile unit is <= t.
 __device__ __forceinline__ uint32_t find_block(const BlockInfo* B, uint32_t b0, uint32_t b1,
@@ -66,21 +66,24 @@ __device__ __forceinline__ uint32_t find_block(const BlockInfo* B, uint32_t b0, 
   return lo;
 }
 
+// vb / vgrid: the workgroup's index among the launch's tile workgroups and
+// their number (a multiple of 8; workgroups before them may do other work,
+// in multiples of 8, so that vb % 8 still names one XCD).
 __device__ __forceinline__ bool tile_of(const BlockInfo* blocks, uint32_t b0, uint32_t b1,
                                         uint32_t n_units, uint32_t mode, uint32_t* bo,
-                                        uint32_t* row0) {
+                                        uint32_t* row0, uint32_t vb, uint32_t vgrid) {
   uint32_t unit, t;
   if (mode == 0) {
-    const uint32_t per = gridDim.x >> 3;
-    unit = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+    const uint32_t per = vgrid >> 3;
+    unit = (vb & 7u) * per + (vb >> 3);
     if (unit >= n_units) return false;
   } else {
-    unit = blockIdx.x >> 3;
+    unit = vb >> 3;
     if (unit >= n_units) return false;
   }
   const uint32_t b = find_block(blocks, b0, b1, unit);
   const uint32_t tiles = (blocks[b].n + kBlkRows - 1) / kBlkRows;
-  t = mode == 0 ? unit - blocks[b].tile0 : 8u * (unit - blocks[b].tile0) + (blockIdx.x & 7u);
+  t = mode == 0 ? unit - blocks[b].tile0 : 8u * (unit - blocks[b].tile0) + (vb & 7u);
   if (t >= tiles) return false;
   *bo = b;
   *row0 = t * kBlkRows;
@@ -177,7 +180,8 @@ blk_round_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_units,
   __shared__ uint32_t s_q[4][kBlkCols];
   __shared__ unsigned long long s_ds[4][kBlkCols];
   uint32_t b, row0;
-  if (!tile_of(A.blocks, b0, b1, n_units, A.tile_mode, &b, &row0)) return;  // uniform per wg
+  if (!tile_of(A.blocks, b0, b1, n_units, A.tile_mode, &b, &row0, blockIdx.x, gridDim.x))
+    return;  // uniform per wg
   const BlockInfo B = A.blocks[b];
   const uint32_t j = lane_id(), wv = wave_id();
   const uint32_t col = b * kBlkCols + j;
@@ -297,6 +301,78 @@ __device__ __forceinline__ uint32_t row_id(__amdgpu_buffer_rsrc_t ars, uint2 r, 
   return __builtin_amdgcn_raw_buffer_load_b32(ars, off, 0, 0) - voff;
 }
 
+// Last block in [b0, b1) whose first hub row is <= t.
+__device__ __forceinline__ uint32_t find_hub_block(const BlockInfo* B, uint32_t b0, uint32_t b1,
+                                                   uint32_t t) {
+  uint32_t lo = b0, hi = b1;
+  while (hi - lo > 1) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (B[mid].hub0 <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// One round of one hub row (degree above the block's hub_deg) of one block,
+// by a whole workgroup: the row's neighbours in 16 contiguous segments, one
+// per quarter-wave (lane ql holding sources 4 ql .. 4 ql + 3), each summed in
+// adjacency order; the 16 segment sums are then added in segment order.  A
+// long row thus costs ~deg / 256 dependent gather steps instead of deg / 16
+// in one quarter-wave.  `unit` indexes the hub rows of the active blocks.
+template <int kChunk>
+__device__ void hub_row(const BlockArgs& A, uint32_t b0, uint32_t b1, uint32_t unit,
+                        const float* __restrict__ cur, float* __restrict__ nxt,
+                        float4 (*part)[16]) {
+  constexpr uint32_t kSeg = 16;
+  const uint32_t b = find_hub_block(A.blocks, b0, b1, unit);
+  const BlockInfo B = A.blocks[b];
+  const uint32_t voff = B.voff;
+  const uint32_t v = A.hubs[A.hub_off[B.gi] + (unit - B.hub0)] - voff;
+  const uint32_t l = lane_id(), wv = wave_id();
+  const uint32_t qd = l >> 4, ql = l & 15u;
+  const uchar4 a4 = *reinterpret_cast<const uchar4*>(A.active + b * kBlkCols + ql * 4u);
+  const bool any = a4.x || a4.y || a4.z || a4.w;
+  if (__ballot(any) == 0ull) return;  // same flags in every wave
+  const uint2 r = A.rowd[voff + v];
+  const uint32_t len = (r.y + kSeg - 1) / kSeg;
+  const uint32_t seg = wv * 4u + qd;
+  const uint32_t k0 = min(r.y, seg * len), k1 = min(r.y, k0 + len);
+  const __amdgpu_buffer_rsrc_t xrs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float*>(cur + B.xoff), 0, static_cast<int>(B.n * (kBlkCols * 4u)), kRsrcWord3);
+  const __amdgpu_buffer_rsrc_t ars = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<uint32_t*>(A.adj), 0, static_cast<int>(A.adj_words * 4u), kRsrcWord3);
+  const uint2 seg_row = make_uint2(r.x, k1);  // ids past the segment read out of range
+  float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  uint32_t my = row_id(ars, seg_row, voff, ql, k0);
+  for (uint32_t k = k0; k < k1; k += kChunk) {
+    const uint32_t my_n = row_id(ars, seg_row, voff, ql, k + kChunk);
+    gather_ids<kChunk>(xrs, my, k1, any, l, k, sum);
+    my = my_n;
+  }
+  part[seg][ql] = sum;
+  __syncthreads();
+  if (wv != 0) return;
+  const uint32_t j = l, col = b * kBlkCols + j;
+  float t = 0.0f;
+#pragma unroll
+  for (uint32_t q = 0; q < kSeg; ++q) {
+    const float4 p4 = part[q][j >> 2];
+    t += (j & 3u) == 0 ? p4.x : ((j & 3u) == 1 ? p4.y : ((j & 3u) == 2 ? p4.z : p4.w));
+  }
+  const uint64_t ro = static_cast<uint64_t>(v) * kBlkCols + j;
+  const bool act = A.active[col] != 0;
+  const float val = act ? t / static_cast<float>(r.y) : cur[B.xoff + ro];
+  nxt[B.xoff + ro] = val;
+  if (act && val > 0.0f) {
+    atomicMax(&A.st_mx[col], __float_as_uint(val));
+    atomicMin(&A.st_mn[col], __float_as_uint(val));
+    atomicAdd(&A.st_q[col], 1u);
+    atomicAdd(&A.st_dsum[col], static_cast<unsigned long long>(r.y));
+  }
+}
+
 // The same round with four rows per wave: each 16-lane quarter takes one
 // row and each lane four sources (16-byte loads), so one load instruction
 // gathers a neighbour row for four vertices at once and a wave keeps up to
@@ -305,15 +381,29 @@ __device__ __forceinline__ uint32_t row_id(__amdgpu_buffer_rsrc_t ars, uint2 r, 
 // as in blk_round_kernel, so both forms give the same bits.
 template <int kChunk, int kWaves>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves)))
-blk_round4_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_units,
+blk_round4_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_units, uint32_t n_hubs,
                   const float* __restrict__ cur, float* __restrict__ nxt) {
   constexpr int kQuarters = 16;  // 4 waves x 4 quarters
   __shared__ float s_mx[kQuarters][kBlkCols];
   __shared__ float s_mn[kQuarters][kBlkCols];
   __shared__ uint32_t s_q[kQuarters][kBlkCols];
   __shared__ unsigned long long s_ds[kQuarters][kBlkCols];
+  __shared__ float4 s_part[16][16];
+  // the first workgroups (n_hubs rounded up to 8) take the hub rows, whose
+  // long chains then overlap the tiles (16-gather form; the 8-gather form
+  // has n_hubs = 0 and blk_hub_kernel after it: the hub path would spill
+  // there)
+  const uint32_t hub_wgs = (n_hubs + 7u) & ~7u;
+  if constexpr (kChunk >= 16) {
+    if (blockIdx.x < hub_wgs) {
+      if (blockIdx.x < n_hubs) hub_row<16>(A, b0, b1, blockIdx.x, cur, nxt, s_part);
+      return;
+    }
+  }
   uint32_t b, row0;
-  if (!tile_of(A.blocks, b0, b1, n_units, A.tile_mode, &b, &row0)) return;  // uniform per wg
+  if (!tile_of(A.blocks, b0, b1, n_units, A.tile_mode, &b, &row0, blockIdx.x - hub_wgs,
+               gridDim.x - hub_wgs))
+    return;  // uniform per wg
   const BlockInfo B = A.blocks[b];
   const uint32_t l = lane_id(), wv = wave_id();
   const uint32_t qd = l >> 4, ql = l & 15u, c0 = ql * 4u;
@@ -363,7 +453,7 @@ blk_round4_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_units,
     if (base >= rend) break;  // uniform
     const uint32_t u = base + qd;
     const uint2 r = rr[g];
-    // rows above the block's hub degree belong to blk_hub_kernel
+    // rows above the block's hub degree belong to the hub workgroups (hub_row)
     const bool rv = u < rend && (B.hub_deg == 0 || r.y <= B.hub_deg);
     const uint32_t my0 = my_next;
     if (g + 1 < kGroups) my_next = row_id(ars, rr[g + 1], voff, ql, 0);
@@ -454,78 +544,12 @@ blk_round4_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_units,
   }
 }
 
-// Last block in [b0, b1) whose first hub row is <= t.
-__device__ __forceinline__ uint32_t find_hub_block(const BlockInfo* B, uint32_t b0, uint32_t b1,
-                                                   uint32_t t) {
-  uint32_t lo = b0, hi = b1;
-  while (hi - lo > 1) {
-    const uint32_t mid = lo + (hi - lo) / 2;
-    if (B[mid].hub0 <= t)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
-// One round of one hub row (degree above the block's hub_deg) of one block per
-// workgroup: the row's neighbours in 16 contiguous segments, one per
-// quarter-wave (lane ql holding sources 4 ql .. 4 ql + 3), each summed in
-// adjacency order; the 16 segment sums are then added in segment order.  A
-// long row thus costs ~deg / 256 dependent gather steps instead of deg / 16
-// in one quarter of blk_round4_kernel, which skips these rows.
+// The hub rows of a round on their own (with the 8-gather round form).
 __global__ void __launch_bounds__(256)
 blk_hub_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_units,
                const float* __restrict__ cur, float* __restrict__ nxt) {
-  constexpr uint32_t kSeg = 16;
-  __shared__ float4 part[kSeg][16];
-  const uint32_t unit = blockIdx.x;
-  if (unit >= n_units) return;
-  const uint32_t b = find_hub_block(A.blocks, b0, b1, unit);
-  const BlockInfo B = A.blocks[b];
-  const uint32_t voff = B.voff;
-  const uint32_t v = A.hubs[A.hub_off[B.gi] + (unit - B.hub0)] - voff;
-  const uint32_t l = lane_id(), wv = wave_id();
-  const uint32_t qd = l >> 4, ql = l & 15u;
-  const uchar4 a4 = *reinterpret_cast<const uchar4*>(A.active + b * kBlkCols + ql * 4u);
-  const bool any = a4.x || a4.y || a4.z || a4.w;
-  if (__ballot(any) == 0ull) return;  // same flags in every wave
-  const uint2 r = A.rowd[voff + v];
-  const uint32_t len = (r.y + kSeg - 1) / kSeg;
-  const uint32_t seg = wv * 4u + qd;
-  const uint32_t k0 = min(r.y, seg * len), k1 = min(r.y, k0 + len);
-  const __amdgpu_buffer_rsrc_t xrs = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<float*>(cur + B.xoff), 0, static_cast<int>(B.n * (kBlkCols * 4u)), kRsrcWord3);
-  const __amdgpu_buffer_rsrc_t ars = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<uint32_t*>(A.adj), 0, static_cast<int>(A.adj_words * 4u), kRsrcWord3);
-  const uint2 seg_row = make_uint2(r.x, k1);  // ids past the segment read out of range
-  float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  uint32_t my = row_id(ars, seg_row, voff, ql, k0);
-  for (uint32_t k = k0; k < k1; k += 16) {
-    const uint32_t my_n = row_id(ars, seg_row, voff, ql, k + 16);
-    gather_ids<16>(xrs, my, k1, any, l, k, sum);
-    my = my_n;
-  }
-  part[seg][ql] = sum;
-  __syncthreads();
-  if (wv != 0) return;
-  const uint32_t j = l, col = b * kBlkCols + j;
-  float t = 0.0f;
-#pragma unroll
-  for (uint32_t q = 0; q < kSeg; ++q) {
-    const float4 p4 = part[q][j >> 2];
-    t += (j & 3u) == 0 ? p4.x : ((j & 3u) == 1 ? p4.y : ((j & 3u) == 2 ? p4.z : p4.w));
-  }
-  const uint64_t ro = static_cast<uint64_t>(v) * kBlkCols + j;
-  const bool act = A.active[col] != 0;
-  const float val = act ? t / static_cast<float>(r.y) : cur[B.xoff + ro];
-  nxt[B.xoff + ro] = val;
-  if (act && val > 0.0f) {
-    atomicMax(&A.st_mx[col], __float_as_uint(val));
-    atomicMin(&A.st_mn[col], __float_as_uint(val));
-    atomicAdd(&A.st_q[col], 1u);
-    atomicAdd(&A.st_dsum[col], static_cast<unsigned long long>(r.y));
-  }
+  __shared__ float4 part[16][16];
+  if (blockIdx.x < n_units) hub_row<16>(A, b0, b1, blockIdx.x, cur, nxt, part);
 }
 
 // After a round: per active source, the round's row update of preds and the
@@ -585,7 +609,8 @@ __global__ void __launch_bounds__(256)
 blk_transpose_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_units) {
   __shared__ float tile[kBlkRows][kBlkCols + 1];
   uint32_t b, row0;
-  if (!tile_of(A.blocks, b0, b1, n_units, A.tile_mode, &b, &row0)) return;  // uniform per wg
+  if (!tile_of(A.blocks, b0, b1, n_units, A.tile_mode, &b, &row0, blockIdx.x, gridDim.x))
+    return;  // uniform per wg
   const BlockInfo B = A.blocks[b];
   const uint32_t L = A.last[b];
   const float* F = ((L & 1u) ? A.xb : A.xa) + B.xoff;
@@ -1078,14 +1103,15 @@ int launch_block_subset(tgtp_ctx* ctx, hipStream_t s, const BlockArgs& A, const 
     float* nxt = (r & 1u) ? A.xb : A.xa;
     const uint32_t tr = round_tiles[r];
     if (tr == 0) continue;
-    if (A.round_form == 1)
-      blk_round4_kernel<8, 6><<<grid_for(tr), 256, 0, s>>>(A, S.b0, S.b1, tr, cur, nxt);
-    else if (A.round_form == 2)
-      blk_round4_kernel<16, 4><<<grid_for(tr), 256, 0, s>>>(A, S.b0, S.b1, tr, cur, nxt);
+    const uint32_t hr = round_hubs[r], hub_wgs = (hr + 7u) & ~7u;
+    if (A.round_form == 1) {
+      blk_round4_kernel<8, 5><<<grid_for(tr), 256, 0, s>>>(A, S.b0, S.b1, tr, 0u, cur, nxt);
+      if (hr) blk_hub_kernel<<<hr, 256, 0, s>>>(A, S.b0, S.b1, hr, cur, nxt);
+    } else if (A.round_form == 2)
+      blk_round4_kernel<16, 4><<<hub_wgs + grid_for(tr), 256, 0, s>>>(A, S.b0, S.b1, tr, hr, cur,
+                                                                       nxt);
     else
       blk_round_kernel<<<grid_for(tr), 256, 0, s>>>(A, S.b0, S.b1, tr, cur, nxt);
-    if (A.round_form != 0 && round_hubs[r] != 0)
-      blk_hub_kernel<<<round_hubs[r], 256, 0, s>>>(A, S.b0, S.b1, round_hubs[r], cur, nxt);
     blk_stop_kernel<<<col_blocks, 256, 0, s>>>(A, S.b0, S.b1, nxt);
   }
   blk_transpose_kernel<<<tile_grid, 256, 0, s>>>(A, S.b0, S.b1, S.tiles);

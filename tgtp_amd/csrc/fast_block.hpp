@@ -67,7 +67,7 @@ struct BlockArgs {
   uint32_t small_bq;   // walk positions with at most this many bouquets: one 8-lane group
   uint32_t round_form; // 0: one row per wave; 1 / 2: four rows per wave, 8 / 16 gathers per step
   uint32_t adj_words;  // CSR entries (the four-rows form needs m < 2^30)
-  // rows of degree above a block's hub_deg are summed by blk_hub_kernel;
+  // rows of degree above a block's hub_deg are summed by hub workgroups;
   // hubs[hub_off[g] ..] lists graph g's, as union vertex ids
   const uint32_t* hubs;
   const uint32_t* hub_off;

@@ -79,7 +79,7 @@ struct FastPlan {
   std::vector<BlockSubset> subsets;
   std::vector<uint32_t> round_tiles;  // per subset, rounds + 1 entries: tiles round r needs
   std::vector<uint32_t> round_hubs;   // ... and hub rows round r needs
-  uint32_t hub_deg = 0;               // rows above this degree go to blk_hub_kernel
+  uint32_t hub_deg = 0;               // rows above this degree go to hub workgroups
   uint32_t* d_hubs = nullptr;         // hub rows (union ids) per graph
   uint32_t* d_hub_off = nullptr;      // [G + 1]
   // scratch
@@ -344,6 +344,8 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
   // otherwise sets the length of every round (n=4096: -17% per score).  On
   // larger graphs the other tiles hide the chains and the extra workgroups
   // cost more than they save (n=28,351, mean degree 34: +10%).
+  // (both four-row forms: a graph's scores must not depend on which form
+  // its plan picked, alone or in a batch)
   P->hub_deg = P->round_form == 0 ? 0u : 128u;
   const int hd = env_int_batch("TGTP_FAST_HUB_DEG", -1);
   if (hd >= 0 && P->round_form != 0) P->hub_deg = static_cast<uint32_t>(hd);
