@@ -292,14 +292,15 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
     TGTP_HIP_CHECK(ctx, hipMemcpyAsync(taus.data(), P->d_taus, dg->m * 4, hipMemcpyDeviceToHost, s));
   TGTP_HIP_CHECK(ctx, hipStreamSynchronize(s));
   // blocks: graphs whose vectors the post kernel reads in place (n >
-  // kPostLds) before those it stages in LDS, so that a subset seldom mixes
-  // the two (a mixed subset takes the staged launch, half as many
-  // workgroups per CU); within each group the largest graphs first
+  // kPostLds), largest m first, then those it stages in LDS by decreasing
+  // n, so that a subset's LDS buffer (sized for its largest staged vector)
+  // fits its graphs closely and a subset seldom mixes the two kinds
   std::vector<uint32_t> order(G);
   std::iota(order.begin(), order.end(), 0u);
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
     const bool la = specs[a].n > kPostLds, lb = specs[b].n > kPostLds;
-    return la != lb ? la : specs[a].m > specs[b].m;
+    if (la != lb) return la;
+    return la ? specs[a].m > specs[b].m : specs[a].n > specs[b].n;
   });
   std::vector<BlockInfo> blocks;
   std::vector<uint32_t> bround;
