@@ -5,14 +5,16 @@
 // shifted per graph), one eigenvector array, per-graph tables.  A FAST plan
 // holds everything a run needs on the device -- per-graph scalars, the
 // padded 16-edge tiles of the tau projection with their column tables, the
-// blocks of 64 sources (largest graphs first) grouped into subsets whose
-// n x 64 diffusion states fit the plan's two state buffers, the round bound
-// of every subset, and all scratch -- so a run is a fixed kernel sequence on
-// one stream: clear, tau projection, per subset (init, rounds, transpose,
-// post), symmetrization.  A single graph is a batch of one, so the plain and
-// batched forms compute the same scores bit for bit.  The batch captures its
-// sequence into a hipGraph once and replays it; many small graphs then share
-// each subset's launches instead of each paying its own.
+// blocks of 64 sources grouped into subsets whose n x 64 diffusion states
+// fit the plan's state buffers (4 GiB each by default), the round bound of
+// every subset, the hub rows of small graphs, and all scratch -- so a run is
+// a fixed kernel sequence: clear, tau projection, per subset (init, rounds,
+// transpose, post), symmetrization, with odd subsets on a second stream
+// (its own buffer pair) when there are several.  A single graph is a batch
+// of one, so the plain and batched forms compute the same scores bit for
+// bit.  The batch captures its sequence into a hipGraph once and replays it;
+// many small graphs then share each subset's launches instead of each
+// paying its own.
 //
 // Autotuning (TGTP_BATCH_AUTOTUNE): coordinate descent over the degree up to
 // which one thread scans a row in the post kernel, the bouquet count below
