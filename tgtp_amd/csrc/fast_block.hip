@@ -578,14 +578,40 @@ blk_stop_kernel(BlockArgs A, uint32_t b0, uint32_t b1, const float* __restrict__
   const float* X = xn + B.xoff;
   const double ps = static_cast<double>(X[static_cast<uint64_t>(s) * kBlkCols + j]);
   double rw = 0.0;
-  for (uint32_t p = r.x + lane_id(); p < r.x + r.y; p += 64) {
-    const uint32_t v = A.adj[p] - B.voff;
-    A.preds_d[p] += (ps - static_cast<double>(X[static_cast<uint64_t>(v) * kBlkCols + j])) / dsd;
-    const int rest = A.taus[p] - ell;
+  auto walk_estimate = [&](int rest) {
     if (rest > 0) {
       const double chi = 2.0 * rest * (static_cast<double>(gmax) - gmin);
       rw += fmax(ceil(8 * (chi * chi) * logmd / dse2), 1.0);
     }
+  };
+  // a lane's positions four at a time (a hub source's row is otherwise a
+  // long chain of dependent loads); the lane's sum keeps position order
+  constexpr uint32_t kU = 4;
+  const uint32_t pend = r.x + r.y;
+  uint32_t p = r.x + lane_id();
+  for (; p + 64u * (kU - 1) < pend; p += 64u * kU) {
+    uint32_t v[kU];
+    int tr[kU];
+    double pd[kU];
+    float xv[kU];
+#pragma unroll
+    for (uint32_t i = 0; i < kU; ++i) {
+      v[i] = A.adj[p + 64u * i] - B.voff;
+      tr[i] = A.taus[p + 64u * i];
+      pd[i] = A.preds_d[p + 64u * i];
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < kU; ++i) xv[i] = X[static_cast<uint64_t>(v[i]) * kBlkCols + j];
+#pragma unroll
+    for (uint32_t i = 0; i < kU; ++i) {
+      A.preds_d[p + 64u * i] = pd[i] + (ps - static_cast<double>(xv[i])) / dsd;
+      walk_estimate(tr[i] - ell);
+    }
+  }
+  for (; p < pend; p += 64) {
+    const uint32_t v = A.adj[p] - B.voff;
+    A.preds_d[p] += (ps - static_cast<double>(X[static_cast<uint64_t>(v) * kBlkCols + j])) / dsd;
+    walk_estimate(A.taus[p] - ell);
   }
   rw = wave_sum(rw);
   if (lane_id() == 0) {
