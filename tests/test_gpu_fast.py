@@ -182,3 +182,17 @@ def test_fast_round_forms_agree(gpu_ctx, monkeypatch):
             d = np.abs(out[0] - seq[0])
             _say(f"  hub rows above degree {hub}: mean change {d.mean():.2e} max {d.max():.2e}")
             assert d.mean() < 1e-3 and d.max() < eps
+
+
+def test_fast_walk_sums_independent_of_small_bq(gpu_ctx, monkeypatch):
+    """A position's 2-way walk sum has one order whichever path runs it (one
+    8-lane group for few bouquets, eight groups for many): scores are equal
+    bit for bit for any small_bq, the knob the batch autotuner sets."""
+    g = G.chung_lu(3000, 2.4, 14.0, seed=21)
+    lam, X = E.eigens_f32(*E.compute_eigens(g, 32))
+    out = []
+    for sbq in ("0", "8", "100000"):
+        monkeypatch.setenv("TGTP_FAST_SMALL_BQ", sbq)
+        out.append(gpu_ctx.score(g, lam, X, 0.01, 10, mode=MODE_FAST_FP32)[0])
+    for s in out[1:]:
+        assert np.array_equal(out[0].view(np.uint32), s.view(np.uint32))

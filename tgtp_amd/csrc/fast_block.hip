@@ -995,9 +995,21 @@ __device__ void post_source(const BlockArgs& A, const BlockInfo& B, uint32_t col
     if (nr == 0 || nr / 8 > small_bq) continue;
     const uint32_t v = A.adj[p];
     const uint32_t step = static_cast<uint32_t>(0xFFFFFFFFull / nr);
+    // bouquet sums grouped by bouquet index mod 8 and the eight partial
+    // sums added in order: the order of the eight-group path below, so a
+    // position's sum does not depend on small_bq (which a batch autotunes)
+    const uint32_t nbq = static_cast<uint32_t>(nr / 8);
+    double part[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (uint32_t j0 = 0; j0 < nbq; j0 += 8) {
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k)
+        if (j0 + k < nbq)
+          part[k] += group_sum8(walk2_bouquet(A, pv, off, src, v, j0 + k, b, step, rest, giw),
+                                gbase);
+    }
     double acc = 0.0;
-    for (uint32_t jb = 0; jb < nr / 8; ++jb)
-      acc += group_sum8(walk2_bouquet(A, pv, off, src, v, jb, b, step, rest, giw), gbase);
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) acc += part[k];
     if (b == 0) {
       A.preds_d[p] += acc / (dsd * static_cast<double>(nr));
       if (A.tr_x) A.tr_x[p] = static_cast<float>(acc);

@@ -336,6 +336,8 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
   // SIMD where rows are long (mean degree >= 8), 8 per step at 6 waves per
   // SIMD on short rows (grids); TGTP_FAST_ROUND=0 selects one row per wave
   P->round_form = dg->m >= 8ull * dg->n ? 2u : 1u;
+  const int sb = env_int_batch("TGTP_FAST_SMALL_BQ", -1);
+  if (sb >= 0) P->small_bq = static_cast<uint32_t>(sb);
   const int rf = env_int_batch("TGTP_FAST_ROUND", -1);
   if (rf >= 0 && rf <= 2) P->round_form = static_cast<uint32_t>(rf);
   // the four-rows form addresses a block's state and the adjacency with
