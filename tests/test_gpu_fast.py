@@ -155,8 +155,8 @@ def test_fast_round_forms_agree(gpu_ctx, monkeypatch):
     same order: without the hub-row kernel the scores are equal bit for
     bit, also with partly filled blocks (n = 5000 leaves 8 sources in the
     last block) and sources that stop at different rounds.  With rows above
-    a degree threshold summed in 16 segments (hub workgroups) the two
-    four-row forms still agree bit for bit; against the sequential sums the
+    a degree threshold summed in 16 segments (hub workgroups) the three
+    forms still agree bit for bit; against the sequential sums the
     diffusion differs by FP32 rounding, which can move a walk count's
     ceiling and so the sampled part of a score: the scores stay close
     (mean change < 1e-3, max < eps)."""
@@ -175,10 +175,11 @@ def test_fast_round_forms_agree(gpu_ctx, monkeypatch):
         for hub in ("128", "6"):  # 6: most rows of these graphs are hubs
             monkeypatch.setenv("TGTP_FAST_HUB_DEG", hub)
             out = []
-            for form in ("1", "2"):
+            for form in ("0", "1", "2"):
                 monkeypatch.setenv("TGTP_FAST_ROUND", form)
                 out.append(gpu_ctx.score(g, lam, X, eps, 10, mode=MODE_FAST_FP32)[0])
-            assert np.array_equal(out[0].view(np.uint32), out[1].view(np.uint32))
+            for s in out[1:]:
+                assert np.array_equal(out[0].view(np.uint32), s.view(np.uint32))
             d = np.abs(out[0] - seq[0])
             _say(f"  hub rows above degree {hub}: mean change {d.mean():.2e} max {d.max():.2e}")
             assert d.mean() < 1e-3 and d.max() < eps

@@ -202,6 +202,8 @@ blk_round_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_units,
   // (Two rows per wave with interleaved gathers measured 8-20% slower.)
   for (uint32_t u = row0 + wu; u < rend; u += 4) {
     const uint64_t ro = static_cast<uint64_t>(u) * kBlkCols + j;
+    // rows above the block's hub degree belong to blk_hub_kernel
+    if (B.hub_deg != 0 && A.rowd[B.voff + u].y > B.hub_deg) continue;  // uniform per wave
     if (act) {
       const uint2 r = A.rowd[B.voff + u];
       const uint32_t start = __builtin_amdgcn_readfirstlane(r.x);
@@ -341,9 +343,11 @@ __device__ void hub_row(const BlockArgs& A, uint32_t b0, uint32_t b1, uint32_t u
   const uint32_t k0 = min(r.y, seg * len), k1 = min(r.y, k0 + len);
   const __amdgpu_buffer_rsrc_t xrs = __builtin_amdgcn_make_buffer_rsrc(
       const_cast<float*>(cur + B.xoff), 0, static_cast<int>(B.n * (kBlkCols * 4u)), kRsrcWord3);
+  // the row's own ids as the buffer (no 32-bit offset limit on the union's
+  // CSR): ids past the segment read out of range
   const __amdgpu_buffer_rsrc_t ars = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<uint32_t*>(A.adj), 0, static_cast<int>(A.adj_words * 4u), kRsrcWord3);
-  const uint2 seg_row = make_uint2(r.x, k1);  // ids past the segment read out of range
+      const_cast<uint32_t*>(A.adj + r.x), 0, static_cast<int>(r.y * 4u), kRsrcWord3);
+  const uint2 seg_row = make_uint2(0u, k1);
   float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   uint32_t my = row_id(ars, seg_row, voff, ql, k0);
   for (uint32_t k = k0; k < k1; k += kChunk) {
@@ -544,7 +548,7 @@ blk_round4_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_units, uint3
   }
 }
 
-// The hub rows of a round on their own (with the 8-gather round form).
+// The hub rows of a round on their own (with the one-row and 8-gather forms).
 __global__ void __launch_bounds__(256)
 blk_hub_kernel(BlockArgs A, uint32_t b0, uint32_t b1, uint32_t n_units,
                const float* __restrict__ cur, float* __restrict__ nxt) {
@@ -1148,8 +1152,10 @@ int launch_block_subset(tgtp_ctx* ctx, hipStream_t s, const BlockArgs& A, const 
     } else if (A.round_form == 2)
       blk_round4_kernel<16, 4><<<hub_wgs + grid_for(tr), 256, 0, s>>>(A, S.b0, S.b1, tr, hr, cur,
                                                                        nxt);
-    else
+    else {
       blk_round_kernel<<<grid_for(tr), 256, 0, s>>>(A, S.b0, S.b1, tr, cur, nxt);
+      if (hr) blk_hub_kernel<<<hr, 256, 0, s>>>(A, S.b0, S.b1, hr, cur, nxt);
+    }
     blk_stop_kernel<<<col_blocks, 256, 0, s>>>(A, S.b0, S.b1, nxt);
   }
   blk_transpose_kernel<<<tile_grid, 256, 0, s>>>(A, S.b0, S.b1, S.tiles);

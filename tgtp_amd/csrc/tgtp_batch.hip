@@ -349,13 +349,14 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
   // otherwise sets the length of every round (n=4096: -17% per score).  On
   // larger graphs the other tiles hide the chains and the extra workgroups
   // cost more than they save (n=28,351, mean degree 34: +10%).
-  // (both four-row forms: a graph's scores must not depend on which form
-  // its plan picked, alone or in a batch)
-  P->hub_deg = P->round_form == 0 ? 0u : 128u;
+  // (in every round form: a graph's scores must not depend on which form
+  // its plan picked, alone or in a batch; hub rows address a block's state
+  // with 32-bit offsets, hence n < 2^24)
+  P->hub_deg = 128u;
   const int hd = env_int_batch("TGTP_FAST_HUB_DEG", -1);
-  if (hd >= 0 && P->round_form != 0) P->hub_deg = static_cast<uint32_t>(hd);
-  const uint32_t hub_max_n =
-      static_cast<uint32_t>(std::max(0, env_int_batch("TGTP_FAST_HUB_MAX_N", 8192)));
+  if (hd >= 0) P->hub_deg = static_cast<uint32_t>(hd);
+  const uint32_t hub_max_n = static_cast<uint32_t>(
+      std::min(std::max(0, env_int_batch("TGTP_FAST_HUB_MAX_N", 8192)), (1 << 24) - 1));
   std::vector<uint32_t> hubs, hub_off(G + 1, 0);
   for (uint32_t k = 0; k < G; ++k) {
     hub_off[k] = static_cast<uint32_t>(hubs.size());
