@@ -489,8 +489,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
-              "design (source sharding + one counter all-reduce) is described in DESIGN.md "
-              "and is not built", file=sys.stderr)
+              "form (sources sharded across GPUs, one all-reduce of the walk counters) "
+              "is not built", file=sys.stderr)
         return 2
     if args.no_saba:  # profiling runs of the tgtp legs only; no headline value
         line = {"metric": METRIC, "value": None, "unit": "transitions/s", "n_gpus": 1}
