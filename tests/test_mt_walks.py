@@ -58,3 +58,42 @@ def test_simd_rejects_partial_bouquets():
     z = load_mt_walk_golden("cycle15")
     with pytest.raises(ValueError):
         oracle.mt_walks(z["idx"], z["adj"], 3, 12, variant=1)
+
+
+def _twist_in_phases(mt, chunk=256):
+    """The GPU kernel's twist (mt_walk.hip): words [0, 227), [227, 454),
+    [454, 623) and 623 as four phases, each in chunks whose reads all happen
+    before the chunk's writes."""
+    N, M = 624, 397
+    up, lo, a = np.uint32(0x80000000), np.uint32(0x7FFFFFFF), np.uint32(0x9908B0DF)
+    for p_lo, p_hi in ((0, N - M), (N - M, 2 * (N - M)), (2 * (N - M), N - 1), (N - 1, N)):
+        for c in range(p_lo, p_hi, chunk):
+            i = np.arange(c, min(c + chunk, p_hi))
+            y = (mt[i] & up) | (mt[(i + 1) % N] & lo)
+            new = mt[(i + M) % N] ^ (y >> np.uint32(1)) ^ np.where(y & np.uint32(1), a, np.uint32(0))
+            mt[i] = new.astype(np.uint32)
+
+
+def _temper(y):
+    y = y ^ (y >> np.uint32(11))
+    y = y ^ ((y << np.uint32(7)) & np.uint32(0x9D2C5680))
+    y = y ^ ((y << np.uint32(15)) & np.uint32(0xEFC60000))
+    return y ^ (y >> np.uint32(18))
+
+
+@pytest.mark.parametrize("seed", [5489, 42, 42 + 4095])
+@pytest.mark.parametrize("chunk", [256, 64])  # the kernel's workgroup, and multi-chunk phases
+def test_phased_twist_matches_std_mt19937(seed, chunk):
+    mt = np.zeros(624, dtype=np.uint32)
+    x = seed
+    mt[0] = x
+    for i in range(1, 624):  # std::mt19937::seed
+        x = (1812433253 * (x ^ (x >> 30)) + i) & 0xFFFFFFFF
+        mt[i] = x
+    outs = []
+    for _ in range(17):
+        _twist_in_phases(mt, chunk)
+        outs.append(_temper(mt.copy()))
+    outs = np.concatenate(outs)
+    for k in (1, 2, 227, 228, 454, 455, 623, 624, 625, 1248, 1249, 5000, 10000):
+        assert int(outs[k - 1]) == oracle.mt19937_nth(seed, k), k
