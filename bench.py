@@ -506,7 +506,7 @@ def main() -> int:
     line = {"metric": METRIC, "value": value, "unit": "transitions/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
-            "data": "synthetic (seeded Chung-Lu generator, regenerated on the box)",
+            "data": "synthetic (seeded Chung-Lu generator, regenerated at run time)",
             "config": cfg, "roofline": roof}
     if not args.no_cpu:
         log("[bench] cpu baseline ...")

@@ -1,10 +1,10 @@
-"""Writes the golden fixtures under tests/golden/ (run in the container that
-holds the reference tree; the GPU box only reads the .npz files).
+"""Writes the golden fixtures under tests/golden/ (run where the reference
+tree is present; the tests themselves only read the .npz files).
 
 Sources of truth:
   * saba_kats.json: known answers of the reference's own saba_randomwalk
-    (src/rw_experiment_saba.cpp:65-145) recorded when it was compiled and run
-    in the survey container: C9 (40 hops, 64 walks) and K5 (20 hops, 64
+    (src/rw_experiment_saba.cpp:65-145) recorded from a build of its own source:
+    C9 (40 hops, 64 walks) and K5 (20 hops, 64
     walks), both forcing the whole-bouquet SplitMix32 fallback.
   * ref_rwmt_<case>.npz: per-vertex visit counts of the reference's own
     scalar_randomwalk (src/rw_experiment.cpp:34-47, per-source std::mt19937
