@@ -38,20 +38,6 @@ constexpr uint32_t kPassU = 8;         // loads per thread in flight in passes o
 constexpr double kMaxWalksFast = 68719476736.0;
 constexpr uint32_t kInfBits = 0x7f800000u;
 
-// Tile of this workgroup (returns false for a workgroup past the last tile;
-// workgroups are dealt to the 8 XCDs round-robin, blockIdx.x % 8).
-//  * tile_mode 0 (blocks of up to 65,536 vertices): XCD x takes the x-th
-//    contiguous eighth of the subset's tiles, so each XCD works through
-//    whole blocks of its own and its L2 holds their hot rows;
-//  * tile_mode 1 (larger blocks, whose n x 64 states would not fit the
-//    Infinity Cache eight at a time): tile groups -- 8 consecutive tiles of
-//    one block, one per XCD -- run in block order, so all XCDs share one
-//    block's state.
-// tile0 of a block counts tiles (mode 0) or tile groups (mode 1) before it.
-__device__ __forceinline__ bool tile_of(const BlockInfo* blocks, uint32_t b0, uint32_t b1,
-                                        uint32_t n_units, uint32_t mode, uint32_t* bo,
-                                        uint32_t* row0, uint32_t vb, uint32_t vgrid);
-
 // Last block in [b0, b1) whose first tile unit is <= t.
 __device__ __forceinline__ uint32_t find_block(const BlockInfo* B, uint32_t b0, uint32_t b1,
                                                uint32_t t) {
@@ -66,6 +52,16 @@ __device__ __forceinline__ uint32_t find_block(const BlockInfo* B, uint32_t b0, 
   return lo;
 }
 
+// Tile of this workgroup (returns false for a workgroup past the last tile;
+// workgroups are dealt to the 8 XCDs round-robin, blockIdx.x % 8).
+//  * tile_mode 0 (blocks of up to 65,536 vertices): XCD x takes the x-th
+//    contiguous eighth of the subset's tiles, so each XCD works through
+//    whole blocks of its own and its L2 holds their hot rows;
+//  * tile_mode 1 (larger blocks, whose n x 64 states would not fit the
+//    Infinity Cache eight at a time): tile groups -- 8 consecutive tiles of
+//    one block, one per XCD -- run in block order, so all XCDs share one
+//    block's state.
+// tile0 of a block counts tiles (mode 0) or tile groups (mode 1) before it.
 // vb / vgrid: the workgroup's index among the launch's tile workgroups and
 // their number (a multiple of 8; workgroups before them may do other work,
 // in multiples of 8, so that vb % 8 still names one XCD).
@@ -344,7 +340,8 @@ __device__ void hub_row(const BlockArgs& A, uint32_t b0, uint32_t b1, uint32_t u
   const __amdgpu_buffer_rsrc_t xrs = __builtin_amdgcn_make_buffer_rsrc(
       const_cast<float*>(cur + B.xoff), 0, static_cast<int>(B.n * (kBlkCols * 4u)), kRsrcWord3);
   // the row's own ids as the buffer (no 32-bit offset limit on the union's
-  // CSR): ids past the segment read out of range
+  // CSR; the host gives hub rows only to graphs whose rows all have fewer
+  // than 2^29 entries): ids past the segment read out of range
   const __amdgpu_buffer_rsrc_t ars = __builtin_amdgcn_make_buffer_rsrc(
       const_cast<uint32_t*>(A.adj + r.x), 0, static_cast<int>(r.y * 4u), kRsrcWord3);
   const uint2 seg_row = make_uint2(0u, k1);

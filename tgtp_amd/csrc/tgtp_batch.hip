@@ -351,7 +351,9 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
   // cost more than they save (n=28,351, mean degree 34: +10%).
   // (in every round form: a graph's scores must not depend on which form
   // its plan picked, alone or in a batch; hub rows address a block's state
-  // with 32-bit offsets, hence n < 2^24)
+  // with 32-bit offsets, hence n < 2^24, and their own row with 32-bit byte
+  // offsets, hence a graph with a row of 2^29 or more entries keeps all its
+  // rows in the round kernels)
   P->hub_deg = 128u;
   const int hd = env_int_batch("TGTP_FAST_HUB_DEG", -1);
   if (hd >= 0) P->hub_deg = static_cast<uint32_t>(hd);
@@ -361,8 +363,13 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
   for (uint32_t k = 0; k < G; ++k) {
     hub_off[k] = static_cast<uint32_t>(hubs.size());
     if (!P->hub_deg || specs[k].n > hub_max_n) continue;
-    for (uint64_t v = specs[k].voff; v < specs[k].voff + specs[k].n; ++v)
-      if (idx[v + 1] - idx[v] > P->hub_deg) hubs.push_back(static_cast<uint32_t>(v));
+    bool short_rows = true;
+    for (uint64_t v = specs[k].voff; v < specs[k].voff + specs[k].n; ++v) {
+      const uint64_t d = idx[v + 1] - idx[v];
+      short_rows = short_rows && d < (1ull << 29);
+      if (d > P->hub_deg) hubs.push_back(static_cast<uint32_t>(v));
+    }
+    if (!short_rows) hubs.resize(hub_off[k]);
   }
   hub_off[G] = static_cast<uint32_t>(hubs.size());
   std::vector<uint32_t> hub_cnt(G);
