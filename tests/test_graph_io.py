@@ -73,6 +73,68 @@ def test_eigen_loader_matches_oracle_and_rejects_nonfinite(tmp_path):
         G.load_eigens_txt(bad, g.n, 12)
 
 
+@pytest.mark.parametrize("text, expect", [
+    ("1.5 0x10", 0.0),         # `in >> x` gathers "0" and stops at 'x' (strtod: 16)
+    ("1.5 0x1p3", 0.0),
+    ("1.5 -.25", -0.25),
+    ("1.5 2.5E+2", 250.0),
+    ("1.5 1e-400", 0.0),       # an underflow keeps 0
+    ("1.5 1e-40", float(np.float32(1e-40))),
+    ("1.5 1e+", None),         # the gathered "1e+" is no number (strtod: 1)
+    ("1.5 1e", None),
+    ("1.5 3.4028236e38", None),  # overflows float
+    ("1.5 inf", None),
+    ("+.e1 2", None),
+])
+def test_eigen_loader_reads_fields_as_istream_does(tmp_path, text, expect):
+    # aesc.h:51-64 reads every field with `in >> x`; the oracle's loader does
+    # the same through libstdc++'s num_get
+    p = tmp_path / "e.txt"
+    p.write_text(text)
+    if expect is None:
+        with pytest.raises(ValueError):
+            oracle.load_eigens(p, 1, 1)
+        with pytest.raises(ValueError, match="invalid eigen"):
+            G.load_eigens_txt(p, 1, 1)
+        return
+    l2, X2 = oracle.load_eigens(p, 1, 1)
+    l1, X1 = G.load_eigens_txt(p, 1, 1)
+    assert l1[0] == l2[0] == 1.5
+    assert X1.tobytes() == X2.tobytes() == np.array([[expect]], np.float32).tobytes()
+
+
+def test_loaders_under_sanitizers_match_oracle(tmp_path):
+    """graph_io.cpp built with AddressSanitizer and UBSan, driven over
+    seeded-random and malformed graph texts, eigenpair files and damaged
+    binary caches (tests/cpp/graph_io_sanitize.cpp)."""
+    import shutil
+    import subprocess
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("g++ not available")
+    root = G.__file__.rsplit("/tgtp_amd/", 1)[0]
+    exe = tmp_path / "graph_io_sanitize"
+    build = subprocess.run(
+        [cxx, "-std=c++20", "-O1", "-g", "-fopenmp", "-fsanitize=address,undefined",
+         "-fno-sanitize-recover=all", f"-I{root}/include",
+         f"{root}/tests/cpp/graph_io_sanitize.cpp", f"{root}/tgtp_amd/csrc/graph_io.cpp",
+         f"{root}/oracle/orc_tgtp.cpp", f"{root}/oracle/orc_walk.cpp", "-o", str(exe)],
+        capture_output=True, text=True)
+    if build.returncode != 0 and "asan" in build.stderr.lower():
+        pytest.skip("sanitizer runtime not available")
+    assert build.returncode == 0, build.stderr[-2000:]
+    out = subprocess.run([str(exe), str(tmp_path), "4000"], capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, (out.stdout + out.stderr)[-4000:]
+    f = out.stdout.split()
+    assert f[0] == "ok" and f[1] == "4000"
+    stats = dict(zip(f[2::2], map(int, f[3::2])))
+    # both outcomes were exercised
+    assert 0 < stats["graphs_parsed"] < 4000
+    assert 0 < stats["eigen_files_parsed"] < 4000
+    assert stats["damaged_caches_refused"] == 4000
+
+
 def _simple_connected(g):
     u = np.repeat(np.arange(g.n), g.degree())
     assert not np.any(u == g.adj), "self-loop"

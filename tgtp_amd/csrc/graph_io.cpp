@@ -3,13 +3,13 @@
 #include "tgtp_graph_io.h"
 
 #include <fcntl.h>
+#include <locale.h>
 #include <omp.h>
 #include <sys/mman.h>
 #include <sys/stat.h>
 #include <unistd.h>
 
 #include <algorithm>
-#include <cerrno>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -17,6 +17,7 @@
 #include <new>
 #include <numeric>
 #include <string>
+#include <type_traits>
 #include <unordered_set>
 #include <utility>
 #include <vector>
@@ -52,6 +53,62 @@ bool parse_i64(const char*& p, const char* end, int64_t& out) {
     ++p;
   }
   out = neg ? static_cast<int64_t>(-static_cast<__int128>(v)) : static_cast<int64_t>(v);
+  return true;
+}
+
+// End of the characters that libstdc++'s `in >> x` gathers for a
+// floating-point x (num_get::_M_extract_float) from p, blanks already
+// skipped: [+-] digits [. digits] and, once a mantissa digit was seen,
+// [eE] [+-] digits.  The value must be the whole of them, so "1e" and "1e+"
+// fail where strtod would read 1, and "0x10" reads 0 and stops at 'x' where
+// strtod would read a hexadecimal 16.
+const char* float_token_end(const char* p, const char* end) {
+  const char* q = p;
+  if (q < end && (*q == '+' || *q == '-')) ++q;
+  bool digits = false;
+  while (q < end && *q >= '0' && *q <= '9') {
+    ++q;
+    digits = true;
+  }
+  if (q < end && *q == '.') {
+    ++q;
+    while (q < end && *q >= '0' && *q <= '9') {
+      ++q;
+      digits = true;
+    }
+  }
+  if (digits && q < end && (*q == 'e' || *q == 'E')) {
+    ++q;
+    if (q < end && (*q == '+' || *q == '-')) ++q;
+    while (q < end && *q >= '0' && *q <= '9') ++q;
+  }
+  return q;
+}
+
+// One floating-point field as `in >> x` reads it in the classic locale:
+// blanks skipped, the gathered characters converted whole (correctly
+// rounded, straight to T), and an overflow (or anything non-finite) is a
+// failure; an underflow keeps strtod's value, as libstdc++ does.  p points
+// into a NUL-terminated buffer ending at `end`; on success it moves past the
+// field.
+template <class T>
+bool parse_float_field(char*& p, char* end, T& out) {
+  static const locale_t c_locale = newlocale(LC_ALL_MASK, "C", static_cast<locale_t>(0));
+  while (p < end && is_space(*p)) ++p;
+  char* q = const_cast<char*>(float_token_end(p, end));
+  if (q == p || !c_locale) return false;
+  const char saved = *q;  // the token is converted on its own
+  *q = '\0';
+  char* next = nullptr;
+  T v;
+  if constexpr (std::is_same_v<T, float>)
+    v = strtof_l(p, &next, c_locale);
+  else
+    v = strtod_l(p, &next, c_locale);
+  *q = saved;
+  if (next != q || !std::isfinite(v)) return false;
+  out = v;
+  p = q;
   return true;
 }
 
@@ -440,27 +497,24 @@ extern "C" int tgtp_load_eigens_txt(const char* path, uint64_t n, int32_t omega,
     set_err(err, errlen, std::string("cannot open eigenpairs: ") + path);
     return -4;
   }
-  // strtod / strtof need a terminated buffer; copy once.
+  // the conversions need a terminated, writable buffer; copy once
   std::string buf(f.data ? f.data : "", f.size);
-  const char* p = buf.c_str();
+  char* p = buf.data();
+  char* const end = p + buf.size();
   for (int32_t i = 0; i < omega; ++i) {
-    char* next = nullptr;
-    errno = 0;
-    const double lam = std::strtod(p, &next);
-    if (next == p || !std::isfinite(lam)) {
+    double lam;
+    if (!parse_float_field(p, end, lam)) {
       set_err(err, errlen, std::string("invalid eigenvalue in ") + path);
       return -1;
     }
     values[i] = lam;
-    p = next;
     for (uint64_t j = 0; j < n; ++j) {
-      const float x = std::strtof(p, &next);  // parsed straight to float, as aesc.h:60
-      if (next == p || !std::isfinite(x)) {
+      float x;  // parsed straight to float, as aesc.h:60
+      if (!parse_float_field(p, end, x)) {
         set_err(err, errlen, std::string("invalid eigenvector in ") + path);
         return -1;
       }
       vectors[static_cast<uint64_t>(omega) * j + i] = x;
-      p = next;
     }
   }
   return 0;
