@@ -46,9 +46,12 @@ void tgtp_csr_free(tgtp_csr* g);
 int tgtp_save_csr_bin(const tgtp_csr* g, const char* path);
 int tgtp_load_csr_bin(const char* path, tgtp_csr* out, char* err, size_t errlen);
 
-/* aesc.h:51-64: omega lines "lambda x_0 ... x_{n-1}", vector entries parsed
- * directly as float; non-finite values are an error (-1), missing file -4.
- * values [omega] double, vectors [n*omega] float vertex-major. */
+/* aesc.h:51-64: omega lines "lambda x_0 ... x_{n-1}", read as omega (n+1)
+ * blank-separated fields exactly as `in >> x` reads them in the classic
+ * locale (eigenvalues as double, entries directly as float; a field that
+ * does not read whole, or overflows, is an error -1, as is a missing field;
+ * missing file -4); parsed by all host threads.  values [omega] double,
+ * vectors [n*omega] float vertex-major. */
 int tgtp_load_eigens_txt(const char* path, uint64_t n, int32_t omega,
                          double* values, float* vectors, char* err,
                          size_t errlen);

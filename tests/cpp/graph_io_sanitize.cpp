@@ -4,7 +4,8 @@
 //     (orc_load_txt, graph.h:29-97): same success, same CSR, or both fail;
 //     many-thread parses of a large file with the same inputs;
 //   * eigenpair files: tgtp_load_eigens_txt against orc_load_eigens
-//     (aesc.h:51-64): same success and bitwise the same values;
+//     (aesc.h:51-64): same success and bitwise the same values, on small
+//     files and on files large enough for a many-thread parse;
 //   * binary CSR caches: every truncation and seeded byte damage of a saved
 //     cache is refused.
 // usage: graph_io_sanitize <dir> <cases>; prints "ok <cases> ..." (with the
@@ -138,6 +139,29 @@ void compare_eigens(const std::string& path, const std::string& text, uint64_t n
   }
 }
 
+// A file of omega (n + 1) fields large enough for a many-thread parse, with
+// up to two odd tokens at random places (inside the fields read, or past
+// them) and sometimes cut short.
+std::string large_eigen_text(std::mt19937_64& rng, uint64_t n, int omega) {
+  const int ntok = static_cast<int>(sizeof(kEigTok) / sizeof(kEigTok[0]));
+  std::vector<std::string> fields;
+  const uint64_t total = (n + 1) * static_cast<uint64_t>(omega) + rng() % 3;
+  char b[64];
+  for (uint64_t i = 0; i < total; ++i) {
+    std::snprintf(b, sizeof(b), "%.9g", std::uniform_real_distribution<double>(-1, 1)(rng));
+    fields.emplace_back(b);
+  }
+  const int odd = static_cast<int>(rng() % 3);
+  for (int k = 0; k < odd; ++k) fields[rng() % fields.size()] = kEigTok[rng() % ntok];
+  std::string s;
+  for (uint64_t i = 0; i < fields.size(); ++i) {
+    s += fields[i];
+    s += (i + 1) % (n + 1) == 0 ? "\n" : (rng() % 50 == 0 ? "\t " : " ");
+  }
+  if (rng() % 4 == 0) s.resize(s.size() - rng() % 64);
+  return s;
+}
+
 void damaged_caches(const std::string& dir, std::mt19937_64& rng, int cases) {
   tgtp_csr g{};
   if (tgtp_gen_chung_lu(300, 2.5, 6.0, 0.0, 11, &g) != 0) {
@@ -210,6 +234,13 @@ int main(int argc, char** argv) {
     const std::string text = random_eigen_text(rng, static_cast<int>(omega * (n + 1)));
     write_file(path, text);
     compare_eigens(path, text, n, omega);
+  }
+  for (int c = 0; c < 16 + cases / 250; ++c) {
+    const uint64_t n = 20000 + rng() % 40000;
+    const int omega = 2 + static_cast<int>(rng() % 3);
+    const std::string text = large_eigen_text(rng, n, omega);
+    write_file(path, text);
+    compare_eigens(path, "<large eigenpair file " + std::to_string(c) + ">", n, omega);
   }
   damaged_caches(dir, rng, cases);
   if (g_fail) {

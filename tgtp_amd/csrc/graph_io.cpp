@@ -59,10 +59,11 @@ bool parse_i64(const char*& p, const char* end, int64_t& out) {
 // End of the characters that libstdc++'s `in >> x` gathers for a
 // floating-point x (num_get::_M_extract_float) from p, blanks already
 // skipped: [+-] digits [. digits] and, once a mantissa digit was seen,
-// [eE] [+-] digits.  The value must be the whole of them, so "1e" and "1e+"
+// [eE] [+-] digits.  The value must be the whole of them (*whole: a mantissa
+// digit and, after an exponent marker, an exponent digit), so "1e" and "1e+"
 // fail where strtod would read 1, and "0x10" reads 0 and stops at 'x' where
 // strtod would read a hexadecimal 16.
-const char* float_token_end(const char* p, const char* end) {
+const char* float_token_end(const char* p, const char* end, bool* whole) {
   const char* q = p;
   if (q < end && (*q == '+' || *q == '-')) ++q;
   bool digits = false;
@@ -77,27 +78,30 @@ const char* float_token_end(const char* p, const char* end) {
       digits = true;
     }
   }
+  *whole = digits;
   if (digits && q < end && (*q == 'e' || *q == 'E')) {
     ++q;
     if (q < end && (*q == '+' || *q == '-')) ++q;
-    while (q < end && *q >= '0' && *q <= '9') ++q;
+    bool exp_digits = false;
+    while (q < end && *q >= '0' && *q <= '9') {
+      ++q;
+      exp_digits = true;
+    }
+    *whole = exp_digits;
   }
   return q;
 }
 
-// One floating-point field as `in >> x` reads it in the classic locale:
-// blanks skipped, the gathered characters converted whole (correctly
-// rounded, straight to T), and an overflow (or anything non-finite) is a
-// failure; an underflow keeps strtod's value, as libstdc++ does.  p points
-// into a NUL-terminated buffer ending at `end`; on success it moves past the
-// field.
+// The gathered field [p, q) converted as `in >> x` converts it in the
+// classic locale: correctly rounded straight to T; an overflow (anything
+// non-finite) fails, an underflow keeps strtod's value as libstdc++ does.
+// *q is set to NUL for the conversion and restored, so [p, q] must be
+// writable and no other thread may read *q meanwhile.
 template <class T>
-bool parse_float_field(char*& p, char* end, T& out) {
+bool convert_float_field(char* p, char* q, T& out) {
   static const locale_t c_locale = newlocale(LC_ALL_MASK, "C", static_cast<locale_t>(0));
-  while (p < end && is_space(*p)) ++p;
-  char* q = const_cast<char*>(float_token_end(p, end));
-  if (q == p || !c_locale) return false;
-  const char saved = *q;  // the token is converted on its own
+  if (!c_locale) return false;
+  const char saved = *q;
   *q = '\0';
   char* next = nullptr;
   T v;
@@ -108,7 +112,6 @@ bool parse_float_field(char*& p, char* end, T& out) {
   *q = saved;
   if (next != q || !std::isfinite(v)) return false;
   out = v;
-  p = q;
   return true;
 }
 
@@ -497,25 +500,93 @@ extern "C" int tgtp_load_eigens_txt(const char* path, uint64_t n, int32_t omega,
     set_err(err, errlen, std::string("cannot open eigenpairs: ") + path);
     return -4;
   }
+  if (omega <= 0) return 0;
   // the conversions need a terminated, writable buffer; copy once
   std::string buf(f.data ? f.data : "", f.size);
-  char* p = buf.data();
-  char* const end = p + buf.size();
-  for (int32_t i = 0; i < omega; ++i) {
-    double lam;
-    if (!parse_float_field(p, end, lam)) {
-      set_err(err, errlen, std::string("invalid eigenvalue in ") + path);
-      return -1;
-    }
-    values[i] = lam;
-    for (uint64_t j = 0; j < n; ++j) {
-      float x;  // parsed straight to float, as aesc.h:60
-      if (!parse_float_field(p, end, x)) {
-        set_err(err, errlen, std::string("invalid eigenvector in ") + path);
-        return -1;
+  char* const data = buf.data();
+  const size_t size = buf.size();
+  // The reference reads omega (n + 1) blank-separated fields in order, each
+  // eigenvalue as double and each entry as float, and stops at the first
+  // that does not read.  Here the file is cut just after blanks (no field
+  // spans a cut, and the sequential read is between fields at each) into
+  // chunks that all host threads parse: pass 1 counts each chunk's fields
+  // up to the first that cannot convert, pass 2 converts each field to the
+  // type its index in the whole file calls for.  The lowest failing index
+  // names the error the sequential read would have met first.
+  const uint64_t per = n + 1, need = per * static_cast<uint64_t>(omega);
+  const int threads = std::max(1, std::min<int>(omp_get_max_threads(),
+                                                static_cast<int>(size >> 18) + 1));
+  std::vector<char*> cut(static_cast<size_t>(threads) + 1, data + size);
+  cut[0] = data;
+  for (int t = 1; t < threads; ++t) {
+    char* c = std::max(data + size / static_cast<size_t>(threads) * static_cast<size_t>(t),
+                       cut[static_cast<size_t>(t) - 1]);
+    while (c < data + size && !is_space(*c)) ++c;
+    cut[static_cast<size_t>(t)] = c < data + size ? c + 1 : data + size;
+  }
+  std::vector<uint64_t> count(static_cast<size_t>(threads), 0), bad(static_cast<size_t>(threads), UINT64_MAX);
+  std::vector<uint8_t> stopped(static_cast<size_t>(threads), 0);
+#pragma omp parallel for schedule(static, 1) num_threads(threads)
+  for (int t = 0; t < threads; ++t) {
+    const char* p = cut[static_cast<size_t>(t)];
+    const char* const e = cut[static_cast<size_t>(t) + 1];
+    uint64_t c = 0;
+    for (;;) {
+      while (p < e && is_space(*p)) ++p;
+      if (p == e) break;
+      bool whole = false;
+      const char* q = float_token_end(p, e, &whole);
+      if (!whole) {
+        stopped[static_cast<size_t>(t)] = 1;
+        break;
       }
-      vectors[static_cast<uint64_t>(omega) * j + i] = x;
+      ++c;
+      p = q;
     }
+    count[static_cast<size_t>(t)] = c;
+  }
+  // index of the first field that cannot be read at all (malformed, or past
+  // the end of the file); chunks after a stopped one do not count
+  std::vector<uint64_t> first(static_cast<size_t>(threads), 0);
+  uint64_t unreadable = 0;
+  bool hit = false;
+  for (int t = 0; t < threads; ++t) {
+    first[static_cast<size_t>(t)] = unreadable;
+    if (hit) continue;
+    unreadable += count[static_cast<size_t>(t)];
+    hit = stopped[static_cast<size_t>(t)] != 0;
+  }
+  const uint64_t lim = std::min(need, unreadable);
+#pragma omp parallel for schedule(static, 1) num_threads(threads)
+  for (int t = 0; t < threads; ++t) {
+    uint64_t idx = first[static_cast<size_t>(t)];
+    char* p = cut[static_cast<size_t>(t)];
+    for (uint64_t k = 0; k < count[static_cast<size_t>(t)] && idx < lim; ++k, ++idx) {
+      while (is_space(*p)) ++p;
+      bool whole = false;
+      char* q = const_cast<char*>(float_token_end(p, cut[static_cast<size_t>(t) + 1], &whole));
+      const uint64_t i = idx / per, r = idx % per;
+      bool ok;
+      if (r == 0) {
+        ok = convert_float_field(p, q, values[i]);
+      } else {
+        // parsed straight to float, as aesc.h:60
+        ok = convert_float_field(p, q, vectors[static_cast<uint64_t>(omega) * (r - 1) + i]);
+      }
+      if (!ok) {
+        bad[static_cast<size_t>(t)] = idx;
+        break;
+      }
+      p = q;
+    }
+  }
+  uint64_t first_bad = unreadable;
+  for (int t = 0; t < threads; ++t) first_bad = std::min(first_bad, bad[static_cast<size_t>(t)]);
+  if (first_bad < need) {
+    set_err(err, errlen, std::string(first_bad % per == 0 ? "invalid eigenvalue in "
+                                                           : "invalid eigenvector in ") +
+                             path);
+    return -1;
   }
   return 0;
 }
