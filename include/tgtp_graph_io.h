@@ -56,6 +56,21 @@ int tgtp_load_eigens_txt(const char* path, uint64_t n, int32_t omega,
                          double* values, float* vectors, char* err,
                          size_t errlen);
 
+/* Binary eigenpair cache -- no reference counterpart (the reference parses
+ * sorted_eigens_<omega>.txt on every run, aesc.h:51-64: about 800 MB of text
+ * at n = 262,144, omega = 128).  Layout, little-endian: "TGTPEIG1", u64 n,
+ * u64 omega, f64 values[omega], f32 vectors[n*omega] vertex-major, u64
+ * digest of the two arrays -- the arrays tgtp_load_eigens_txt returns.
+ * Save refuses non-finite values (-1) and writes a temporary file then
+ * renames it (-4 write failure).  Load takes the n and omega the caller
+ * expects: a file of other sizes, damaged, or holding non-finite values is
+ * -1; -4 cannot open. */
+int tgtp_save_eigens_bin(const char* path, uint64_t n, int32_t omega,
+                         const double* values, const float* vectors);
+int tgtp_load_eigens_bin(const char* path, uint64_t n, int32_t omega,
+                         double* values, float* vectors, char* err,
+                         size_t errlen);
+
 /* aesc.h:87-89: 2 <= omega <= n and no vertex of degree 0.  0 or -1. */
 int tgtp_check_aesc_graph(const tgtp_csr* g, int32_t omega, char* err,
                           size_t errlen);

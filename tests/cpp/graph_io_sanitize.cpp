@@ -6,8 +6,8 @@
 //   * eigenpair files: tgtp_load_eigens_txt against orc_load_eigens
 //     (aesc.h:51-64): same success and bitwise the same values, on small
 //     files and on files large enough for a many-thread parse;
-//   * binary CSR caches: every truncation and seeded byte damage of a saved
-//     cache is refused.
+//   * binary CSR and eigenpair caches: every truncation and seeded byte
+//     damage of a saved cache is refused.
 // usage: graph_io_sanitize <dir> <cases>; prints "ok <cases> ..." (with the
 // number of inputs each loader accepted or refused) on success.
 #include <omp.h>
@@ -26,7 +26,7 @@
 namespace {
 
 int g_fail = 0;
-int g_graphs_ok = 0, g_eigens_ok = 0, g_caches_refused = 0;
+int g_graphs_ok = 0, g_eigens_ok = 0, g_caches_refused = 0, g_eig_caches_refused = 0;
 
 void fail(const std::string& what, const std::string& input) {
   if (g_fail++ < 10) std::printf("MISMATCH %s\n--- input ---\n%s\n---\n", what.c_str(), input.c_str());
@@ -204,6 +204,46 @@ void damaged_caches(const std::string& dir, std::mt19937_64& rng, int cases) {
   tgtp_csr_free(&g);
 }
 
+void damaged_eigen_caches(const std::string& dir, std::mt19937_64& rng, int cases) {
+  const uint64_t n = 300;
+  const int omega = 8;
+  std::vector<double> lam(omega);
+  std::vector<float> X(n * omega);
+  for (auto& v : lam) v = std::uniform_real_distribution<double>(-1, 1)(rng);
+  for (auto& v : X) v = std::uniform_real_distribution<float>(-1, 1)(rng);
+  const std::string path = dir + "/e.bin";
+  if (tgtp_save_eigens_bin(path.c_str(), n, omega, lam.data(), X.data()) != 0) {
+    fail("eigen cache: save", "");
+    return;
+  }
+  std::ifstream in(path, std::ios::binary);
+  const std::string good((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+  std::vector<double> l2(omega);
+  std::vector<float> X2(n * omega);
+  char err[256];
+  if (tgtp_load_eigens_bin(path.c_str(), n, omega, l2.data(), X2.data(), err, sizeof(err)) != 0 ||
+      l2 != lam || X2 != X)
+    fail("eigen cache: round trip", "");
+  if (tgtp_load_eigens_bin(path.c_str(), n + 1, omega, l2.data(), X2.data(), err, sizeof(err)) == 0)
+    fail("eigen cache: accepted for another n", "");
+  const std::string bad = dir + "/bad.bin";
+  for (int c = 0; c < cases; ++c) {
+    std::string s = good;
+    if (c % 3 == 0) {
+      s.resize(rng() % s.size());
+    } else {
+      const int flips = 1 + static_cast<int>(rng() % 3);
+      for (int f = 0; f < flips; ++f) s[rng() % s.size()] ^= static_cast<char>(1 + rng() % 255);
+    }
+    if (s == good) continue;
+    write_file(bad, s);
+    if (tgtp_load_eigens_bin(bad.c_str(), n, omega, l2.data(), X2.data(), err, sizeof(err)) == 0)
+      fail("eigen cache: damaged file accepted (case " + std::to_string(c) + ")", "");
+    else
+      ++g_eig_caches_refused;
+  }
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -243,11 +283,14 @@ int main(int argc, char** argv) {
     compare_eigens(path, "<large eigenpair file " + std::to_string(c) + ">", n, omega);
   }
   damaged_caches(dir, rng, cases);
+  damaged_eigen_caches(dir, rng, cases);
   if (g_fail) {
     std::printf("%d mismatches\n", g_fail);
     return 1;
   }
-  std::printf("ok %d graphs_parsed %d eigen_files_parsed %d damaged_caches_refused %d\n", cases,
-              g_graphs_ok, g_eigens_ok, g_caches_refused);
+  std::printf(
+      "ok %d graphs_parsed %d eigen_files_parsed %d damaged_caches_refused %d "
+      "damaged_eigen_caches_refused %d\n",
+      cases, g_graphs_ok, g_eigens_ok, g_caches_refused, g_eig_caches_refused);
   return 0;
 }

@@ -77,6 +77,30 @@ def test_tgtp_cli_scores_bin_matches_oracle(tmp_path):
 
 
 @pytest.mark.gpu
+def test_tgtp_cli_binary_caches(tmp_path):
+    """TGTP_CACHE=1: the first run writes graph.txt.csr and
+    sorted_eigens_<omega>.txt.bin, later runs read them; the scores are the
+    same bits as a run from the text alone."""
+    import os
+    g = G.chung_lu(500, 2.5, 9.0, seed=15)
+    _dataset(tmp_path, "cl500", g, 16)
+    d = tmp_path / "cl500"
+    plain, first, second = (tmp_path / f"{k}.bin" for k in ("plain", "first", "second"))
+    args = [BIN / "tgtp", tmp_path, "cl500", "0.05", "16", "10"]
+    r = run(args + [plain])
+    assert r.returncode == 0, r.stderr
+    assert not (d / "graph.txt.csr").exists()
+    env = dict(os.environ, TGTP_CACHE="1")
+    for out in (first, second):
+        r = subprocess.run([str(a) for a in args + [out]], capture_output=True, text=True, env=env)
+        assert r.returncode == 0, r.stderr
+        assert (d / "graph.txt.csr").exists() and (d / "sorted_eigens_16.txt.bin").exists()
+    assert plain.read_bytes() == first.read_bytes() == second.read_bytes()
+    h = G.load_bin(d / "graph.txt.csr")
+    assert np.array_equal(h.idx, g.idx) and np.array_equal(h.adj, g.adj)
+
+
+@pytest.mark.gpu
 def test_benchmark_cli_rows(tmp_path):
     g = G.chung_lu(600, 2.5, 9.0, seed=13)
     _dataset(tmp_path, "cl600", g, 16)

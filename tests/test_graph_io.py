@@ -133,6 +133,7 @@ def test_loaders_under_sanitizers_match_oracle(tmp_path):
     assert 0 < stats["graphs_parsed"] < 4000
     assert 0 < stats["eigen_files_parsed"] < 4000
     assert stats["damaged_caches_refused"] == 4000
+    assert stats["damaged_eigen_caches_refused"] == 4000
 
 
 def _simple_connected(g):
@@ -261,6 +262,64 @@ def test_csr_binary_cache_checks_invariants(tmp_path):
     G.save_bin(G.Graph(g.idx, adj), tmp_path / "u.csr")
     with pytest.raises(ValueError, match="invariants"):
         G.load_bin(tmp_path / "u.csr")
+
+
+def test_eigen_binary_cache_round_trip_and_damage(tmp_path):
+    g = G.chung_lu(300, 2.5, 8.0, seed=4)
+    lam, X = E.eigens_f32(*E.compute_eigens(g, 16))
+    p = tmp_path / "e.bin"
+    G.save_eigens_bin(p, lam, X)
+    l2, X2 = G.load_eigens_bin(p, g.n, 16)
+    assert l2.tobytes() == lam.tobytes() and X2.tobytes() == X.tobytes()
+    # the caller's sizes must match the file's
+    for n, om in ((g.n - 1, 16), (g.n, 15), (g.n + 1, 16)):
+        with pytest.raises(ValueError, match="expected n"):
+            G.load_eigens_bin(p, n, om)
+    raw = bytearray(p.read_bytes())
+    for pos in (3, 12, 30, len(raw) // 2, len(raw) - 3):  # magic, header, values, vectors, digest
+        bad = bytearray(raw)
+        bad[pos] ^= 0x10
+        q = tmp_path / f"bad{pos}.bin"
+        q.write_bytes(bytes(bad))
+        with pytest.raises(ValueError, match="not a valid eigenpair cache"):
+            G.load_eigens_bin(q, g.n, 16)
+    q = tmp_path / "short.bin"
+    q.write_bytes(bytes(raw[:-1]))
+    with pytest.raises(ValueError):
+        G.load_eigens_bin(q, g.n, 16)
+    with pytest.raises(OSError):
+        G.load_eigens_bin(tmp_path / "missing.bin", g.n, 16)
+    Xn = X.copy()
+    Xn[5, 3] = np.nan
+    with pytest.raises(ValueError, match="finite"):
+        G.save_eigens_bin(tmp_path / "nan.bin", lam, Xn)
+    assert not (tmp_path / "nan.bin").exists()
+
+
+def test_load_eigens_txt_cached(tmp_path):
+    import os
+    g = G.chung_lu(200, 2.5, 8.0, seed=4)
+    lam, X = E.eigens_f32(*E.compute_eigens(g, 12))
+    p = tmp_path / "sorted_eigens_12.txt"
+    E.write_eigens_txt(p, lam, X)
+    l1, X1 = G.load_eigens_txt_cached(p, g.n, 12)
+    cache = tmp_path / "sorted_eigens_12.txt.bin"
+    assert cache.exists()
+    assert l1.tobytes() == lam.tobytes() and X1.tobytes() == X.tobytes()
+    # a cache holding other values is used while it is newer than the text
+    G.save_eigens_bin(cache, lam * 2, X)
+    l2, _ = G.load_eigens_txt_cached(p, g.n, 12)
+    assert np.array_equal(l2, lam * 2)
+    # ... and rebuilt once the text is newer
+    st = cache.stat()
+    os.utime(p, (st.st_atime + 10, st.st_mtime + 10))
+    l3, X3 = G.load_eigens_txt_cached(p, g.n, 12)
+    assert l3.tobytes() == lam.tobytes() and X3.tobytes() == X.tobytes()
+    # a cache of other sizes is rebuilt, not trusted
+    os.utime(cache, (st.st_atime + 20, st.st_mtime + 20))
+    G.save_eigens_bin(cache, lam[:11], X[:, :11])
+    l4, _ = G.load_eigens_txt_cached(p, g.n, 12)
+    assert l4.tobytes() == lam.tobytes()
 
 
 def test_load_txt_cached(tmp_path):

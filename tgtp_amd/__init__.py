@@ -11,5 +11,5 @@ Layers:
 from ._lib import (MODE_FAITHFUL, MODE_FAST_BF16, MODE_FAST_FP32, WALK_MT_REGULAR,  # noqa: F401
                    WALK_MT_SIMD, TgtpError)
 from .graph import (Graph, chung_lu, complete, cycle, grid_rewired, load_eigens_txt,  # noqa: F401
-                    load_txt, petersen)
+                    load_eigens_txt_cached, load_txt, load_txt_cached, petersen)
 from .runtime import Context, DeviceGraph  # noqa: F401

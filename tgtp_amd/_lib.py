@@ -75,8 +75,8 @@ EXPORTED = [
     "tgtp_batch_create", "tgtp_batch_run", "tgtp_batch_destroy", "tgtp_batch_config",
     "tgtp_score_batch",
     "tgtp_load_txt", "tgtp_csr_free", "tgtp_save_csr_bin", "tgtp_load_csr_bin",
-    "tgtp_load_eigens_txt", "tgtp_check_aesc_graph",
-    "tgtp_gen_chung_lu", "tgtp_gen_grid_rewired", "tgtp_write_graph_txt",
+    "tgtp_load_eigens_txt", "tgtp_save_eigens_bin", "tgtp_load_eigens_bin",
+    "tgtp_check_aesc_graph", "tgtp_gen_chung_lu", "tgtp_gen_grid_rewired", "tgtp_write_graph_txt",
     "tgtp_csr_reverse_index",
 ]
 
@@ -141,6 +141,10 @@ def lib() -> C.CDLL:
     L.tgtp_csr_free.argtypes = [P(CsrStruct)]
     L.tgtp_csr_free.restype = None
     L.tgtp_load_eigens_txt.argtypes = [C.c_char_p, C.c_uint64, C.c_int32, P(C.c_double),
+                                       P(C.c_float), C.c_char_p, C.c_size_t]
+    L.tgtp_save_eigens_bin.argtypes = [C.c_char_p, C.c_uint64, C.c_int32, P(C.c_double),
+                                       P(C.c_float)]
+    L.tgtp_load_eigens_bin.argtypes = [C.c_char_p, C.c_uint64, C.c_int32, P(C.c_double),
                                        P(C.c_float), C.c_char_p, C.c_size_t]
     L.tgtp_check_aesc_graph.argtypes = [P(CsrStruct), C.c_int32, C.c_char_p, C.c_size_t]
     L.tgtp_gen_chung_lu.argtypes = [C.c_uint64, C.c_double, C.c_double, C.c_double,

@@ -45,8 +45,7 @@ int main(int argc, char** argv) {
     std::vector<double> lam(omega);
     std::vector<float> X(csr.g.n * static_cast<uint64_t>(omega));
     const std::string epath = (dir / ("sorted_eigens_" + std::to_string(omega) + ".txt")).string();
-    if (tgtp_load_eigens_txt(epath.c_str(), csr.g.n, omega, lam.data(), X.data(), err, sizeof(err)))
-      throw std::runtime_error(err);
+    cli::load_eigens(epath, csr.g.n, omega, lam.data(), X.data());
     cli::Ctx ctx(cli::device_from_env());
     const tgtp_graph gv = csr.view();
     tgtp_dgraph* dg = nullptr;

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
+#include <filesystem>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -60,10 +61,46 @@ struct Csr {
   tgtp_graph view() const { return tgtp_graph{g.n, g.m, g.idx, g.adj}; }
 };
 
+// TGTP_CACHE=1: text inputs go through binary caches beside them
+// (<graph.txt>.csr, <eigens.txt>.bin), used while newer than the text and
+// whole, rebuilt otherwise (a cache that cannot be written is skipped).
+inline bool cache_from_env() {
+  const char* s = std::getenv("TGTP_CACHE");
+  return s && std::string(s) == "1";
+}
+
+inline bool cache_fresh(const std::string& text, const std::string& cache) {
+  std::error_code ec;
+  const auto tc = std::filesystem::last_write_time(cache, ec);
+  if (ec) return false;
+  const auto tt = std::filesystem::last_write_time(text, ec);
+  return !ec && tc >= tt;
+}
+
 inline void load_graph(const std::string& path, Csr& out) {
   char err[512] = {0};
+  const std::string cache = path + ".csr";
+  const bool use_cache = cache_from_env();
+  if (use_cache && cache_fresh(path, cache) &&
+      tgtp_load_csr_bin(cache.c_str(), &out.g, err, sizeof(err)) == 0)
+    return;
   const int rc = tgtp_load_txt(path.c_str(), &out.g, err, sizeof(err));
   if (rc) throw std::runtime_error(err);
+  if (use_cache) tgtp_save_csr_bin(&out.g, cache.c_str());
+}
+
+// values [omega], vectors [n * omega] vertex-major (aesc.h:51-64)
+inline void load_eigens(const std::string& path, uint64_t n, int32_t omega, double* values,
+                        float* vectors) {
+  char err[512] = {0};
+  const std::string cache = path + ".bin";
+  const bool use_cache = cache_from_env();
+  if (use_cache && cache_fresh(path, cache) &&
+      tgtp_load_eigens_bin(cache.c_str(), n, omega, values, vectors, err, sizeof(err)) == 0)
+    return;
+  if (tgtp_load_eigens_txt(path.c_str(), n, omega, values, vectors, err, sizeof(err)))
+    throw std::runtime_error(err);
+  if (use_cache) tgtp_save_eigens_bin(cache.c_str(), n, omega, values, vectors);
 }
 
 struct Ctx {

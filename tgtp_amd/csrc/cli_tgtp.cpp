@@ -2,7 +2,8 @@
 // <folder>/<graph>/graph.txt and sorted_eigens_<omega>.txt, sets
 // delta = 1/n, writes headerless float32 scores in CSR order (scores.bin).
 // The scores come from tgtp_score on the GPU; TGTP_MODE selects the
-// arithmetic (faithful by default, bit-exact with the reference).
+// arithmetic (faithful by default, bit-exact with the reference);
+// TGTP_CACHE=1 reads both text inputs through binary caches beside them.
 //
 // usage: tgtp <folder> <graph> <epsilon> <omega> <gamma> <outfile>
 #include <climits>
@@ -40,8 +41,7 @@ int main(int argc, char** argv) {
     std::vector<double> lam(om);
     std::vector<float> X(csr.g.n * static_cast<uint64_t>(om));
     const std::string epath = (dir / ("sorted_eigens_" + std::to_string(om) + ".txt")).string();
-    if (tgtp_load_eigens_txt(epath.c_str(), csr.g.n, om, lam.data(), X.data(), err, sizeof(err)))
-      throw std::runtime_error(err);
+    cli::load_eigens(epath, csr.g.n, om, lam.data(), X.data());
     std::ofstream out(argv[6], std::ios::binary);
     if (!out) throw std::runtime_error("cannot open output file");
     cli::Ctx ctx(cli::device_from_env());

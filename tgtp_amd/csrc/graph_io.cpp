@@ -237,6 +237,7 @@ uint64_t digest_words(const void* p, size_t bytes, uint64_t h) {
 }
 
 constexpr char kCsrMagic[8] = {'T', 'G', 'T', 'P', 'C', 'S', 'R', '1'};
+constexpr char kEigMagic[8] = {'T', 'G', 'T', 'P', 'E', 'I', 'G', '1'};
 
 // Builds a sorted-row CSR holding both directions of each (a, b) pair.
 int build_csr(uint64_t n, const std::vector<std::pair<uint32_t, uint32_t>>& edges,
@@ -489,6 +490,82 @@ extern "C" int tgtp_load_csr_bin(const char* path, tgtp_csr* out, char* err, siz
   out->m = m;
   out->idx = idx;
   out->adj = adj;
+  return 0;
+}
+
+extern "C" int tgtp_save_eigens_bin(const char* path, uint64_t n, int32_t omega,
+                                    const double* values, const float* vectors) {
+  if (!path || omega <= 0 || !values || (n && !vectors)) return -1;
+  const uint64_t om = static_cast<uint64_t>(omega), cnt = n * om;
+  for (uint64_t i = 0; i < om; ++i)
+    if (!std::isfinite(values[i])) return -1;
+  int nonfinite = 0;
+#pragma omp parallel for schedule(static) reduction(+ : nonfinite)
+  for (int64_t i = 0; i < static_cast<int64_t>(cnt); ++i) nonfinite += !std::isfinite(vectors[i]);
+  if (nonfinite) return -1;
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE* fp = std::fopen(tmp.c_str(), "wb");
+  if (!fp) return -4;
+  const uint64_t hdr[2] = {n, om};
+  const size_t vb = om * sizeof(double), xb = cnt * sizeof(float);
+  uint64_t h = digest_words(values, vb, 0xbb67ae8584caa73bull ^ n);
+  h = digest_words(vectors, xb, h ^ om);
+  bool ok = std::fwrite(kEigMagic, 1, 8, fp) == 8 && std::fwrite(hdr, 8, 2, fp) == 2 &&
+            std::fwrite(values, 1, vb, fp) == vb && (xb == 0 || std::fwrite(vectors, 1, xb, fp) == xb) &&
+            std::fwrite(&h, 8, 1, fp) == 1;
+  ok = (std::fclose(fp) == 0) && ok;
+  if (!ok || std::rename(tmp.c_str(), path) != 0) {
+    std::remove(tmp.c_str());
+    return -4;
+  }
+  return 0;
+}
+
+extern "C" int tgtp_load_eigens_bin(const char* path, uint64_t n, int32_t omega, double* values,
+                                    float* vectors, char* err, size_t errlen) {
+  if (!path || omega <= 0 || !values || (n && !vectors)) {
+    set_err(err, errlen, "bad argument");
+    return -1;
+  }
+  Mapped f;
+  if (!f.open_file(path)) {
+    set_err(err, errlen, std::string("cannot open eigenpair cache: ") + path);
+    return -4;
+  }
+  const std::string bad = std::string("not a valid eigenpair cache: ") + path;
+  const uint64_t om = static_cast<uint64_t>(omega);
+  // sizes first, so that a foreign or truncated file is refused before any
+  // product of its header is formed
+  uint64_t hdr[2] = {0, 0};
+  if (f.size >= 24) std::memcpy(hdr, f.data + 8, 16);
+  if (f.size < 24 || std::memcmp(f.data, kEigMagic, 8) != 0 || hdr[0] != n || hdr[1] != om ||
+      n > (UINT64_MAX / 4 - 64) / om || f.size != 24 + om * 8 + n * om * 4 + 8) {
+    set_err(err, errlen, bad + " (expected n = " + std::to_string(n) +
+                             ", omega = " + std::to_string(omega) + ")");
+    return -1;
+  }
+  const char* pv = f.data + 24;
+  const char* px = pv + om * 8;
+  const size_t xb = n * om * 4;
+  uint64_t h = digest_words(pv, om * 8, 0xbb67ae8584caa73bull ^ n);
+  h = digest_words(px, xb, h ^ om);
+  uint64_t stored;
+  std::memcpy(&stored, px + xb, 8);
+  if (h != stored) {
+    set_err(err, errlen, bad + " (checksum)");
+    return -1;
+  }
+  std::memcpy(values, pv, om * 8);
+  if (xb) std::memcpy(vectors, px, xb);
+  bool finite = true;
+  for (uint64_t i = 0; i < om; ++i) finite = finite && std::isfinite(values[i]);
+  int nonfinite = 0;
+#pragma omp parallel for schedule(static) reduction(+ : nonfinite)
+  for (int64_t i = 0; i < static_cast<int64_t>(n * om); ++i) nonfinite += !std::isfinite(vectors[i]);
+  if (!finite || nonfinite) {
+    set_err(err, errlen, bad + " (non-finite values)");
+    return -1;
+  }
   return 0;
 }
 

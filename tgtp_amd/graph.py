@@ -165,6 +165,50 @@ def load_eigens_txt(path, n: int, omega: int):
     return lam, X
 
 
+def save_eigens_bin(path, values: np.ndarray, vectors: np.ndarray) -> None:
+    """Binary eigenpair cache (tgtp_save_eigens_bin): values f64[omega],
+    vectors f32[n, omega] vertex-major, as load_eigens_txt returns them."""
+    lam = np.ascontiguousarray(values, dtype=np.float64)
+    X = np.ascontiguousarray(vectors, dtype=np.float32)
+    if X.ndim != 2 or X.shape[1] != lam.shape[0]:
+        raise ValueError("vectors must be [n, omega] for omega values")
+    rc = L.lib().tgtp_save_eigens_bin(str(path).encode(), X.shape[0], lam.shape[0],
+                                      L.ptr(lam, C.c_double), L.ptr(X, C.c_float))
+    if rc == L.TGTP_E_IO:
+        raise OSError(f"cannot write {path}")
+    if rc != 0:
+        raise ValueError("eigenpairs must be finite")
+
+
+def load_eigens_bin(path, n: int, omega: int):
+    """tgtp_load_eigens_bin: the cache must hold exactly n x omega pairs."""
+    lam = np.empty(omega, dtype=np.float64)
+    X = np.empty((n, omega), dtype=np.float32)
+    err = C.create_string_buffer(512)
+    rc = L.lib().tgtp_load_eigens_bin(str(path).encode(), n, omega, L.ptr(lam, C.c_double),
+                                      L.ptr(X, C.c_float), err, 512)
+    if rc == L.TGTP_E_IO:
+        raise OSError(err.value.decode())
+    if rc != 0:
+        raise ValueError(err.value.decode())
+    return lam, X
+
+
+def load_eigens_txt_cached(path, n: int, omega: int, cache=None):
+    """load_eigens_txt through a binary cache (default: path + ".bin"),
+    rebuilt when missing, older than the text, damaged or of other sizes."""
+    path = Path(path)
+    cache = Path(cache) if cache is not None else path.with_name(path.name + ".bin")
+    if cache.exists() and cache.stat().st_mtime >= path.stat().st_mtime:
+        try:
+            return load_eigens_bin(cache, n, omega)
+        except ValueError:
+            pass
+    lam, X = load_eigens_txt(path, n, omega)
+    save_eigens_bin(cache, lam, X)
+    return lam, X
+
+
 def check_aesc_graph(g: Graph, omega: int) -> None:
     """AESCGraph constructor checks (src/aesc.h:87-89)."""
     cs = g.csr_struct()
