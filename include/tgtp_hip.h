@@ -130,7 +130,9 @@ int tgtp_saba_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,
  * i*depth + h; simd: bouquets of 8 walks, hop h of bouquet b takes draws
  * b*8*depth + 8h + lane.  Counters and sums as tgtp_saba_walks.  simd needs
  * n_walks % 8 == 0 (TGTP_E_ARG otherwise); depth up to 3472 (regular) / 434
- * (simd), TGTP_E_UNSUPPORTED beyond. */
+ * (simd), TGTP_E_UNSUPPORTED beyond.  In a non-symmetric CSR a walker that
+ * reaches a vertex without neighbours stops there (the reference divides by
+ * zero); the draw indices of all other walks are unchanged. */
 enum { TGTP_WALK_MT_REGULAR = 0, TGTP_WALK_MT_SIMD = 1 };
 int tgtp_mt_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,
                   uint32_t src_end, uint32_t depth, uint32_t n_walks, int32_t variant,

@@ -207,12 +207,18 @@ extern "C" int orc_mt_walks(uint64_t n, const uint32_t* idx, const uint32_t* adj
       const uint32_t src = static_cast<uint32_t>(s);
       if (deg(idx, src) == 0) continue;
       rng.seed(42u + src);
+      // A walker at a vertex without neighbours (reachable only in a
+      // non-symmetric CSR; the reference divides by zero there) stays and
+      // stops counting; its draws are still taken, so every other walk
+      // keeps its place in the stream.
       if (variant == 0) {
         for (uint64_t i = 0; i < n_walks; ++i) {
           uint32_t cur = src;
           for (uint32_t h = 0; h < depth; ++h) {
-            const uint32_t k = rng() % deg(idx, cur);
-            cur = adj[idx[cur] + k];
+            const uint32_t r = rng();
+            const uint32_t d = deg(idx, cur);
+            if (d == 0) continue;
+            cur = adj[idx[cur] + r % d];
             ++local[cur];
           }
         }
@@ -223,8 +229,12 @@ extern "C" int orc_mt_walks(uint64_t n, const uint32_t* idx, const uint32_t* adj
           for (uint32_t h = 0; h < depth; ++h) {
             uint32_t rnd[8];
             for (int j = 0; j < 8; ++j) rnd[j] = rng();
-            for (int j = 0; j < 8; ++j) cur[j] = adj[idx[cur[j]] + rnd[j] % deg(idx, cur[j])];
-            for (int j = 0; j < 8; ++j) ++local[cur[j]];
+            for (int j = 0; j < 8; ++j) {
+              const uint32_t d = deg(idx, cur[j]);
+              if (d == 0) continue;
+              cur[j] = adj[idx[cur[j]] + rnd[j] % d];
+              ++local[cur[j]];
+            }
           }
         }
       }

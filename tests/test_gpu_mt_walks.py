@@ -94,3 +94,22 @@ def test_errors(gpu_ctx):
             gpu_ctx.mt_walks(dg, 3, 8, WALK_MT_REGULAR, 0, 10)
     finally:
         dg.free()
+
+
+@pytest.mark.parametrize("variant", [WALK_MT_REGULAR, WALK_MT_SIMD])
+def test_walker_stops_at_vertex_without_neighbours(gpu_ctx, variant):
+    """A non-symmetric CSR through the C ABI: walkers that reach vertex 2
+    (no neighbours) stop there, as in the oracle (itself pinned by a
+    brute-force restatement, tests/test_mt_walks.py)."""
+    from test_mt_walks import SINK_ADJ, SINK_IDX
+    g = G.Graph(SINK_IDX, SINK_ADJ)
+    dg = gpu_ctx.upload(g)
+    try:
+        for depth, n_walks in ((7, 24), (40, 256)):
+            _say(f"mt variant {variant} sink graph: depth={depth} W={n_walks}")
+            cnt, vc, cs, _ = gpu_ctx.mt_walks(dg, depth, n_walks, variant)
+            ref = oracle.mt_walks(SINK_IDX, SINK_ADJ, depth, n_walks, variant=variant)
+            assert np.array_equal(cnt, ref)
+            assert (vc, cs) == _sums(ref)
+    finally:
+        dg.free()

@@ -97,3 +97,52 @@ def test_phased_twist_matches_std_mt19937(seed, chunk):
     outs = np.concatenate(outs)
     for k in (1, 2, 227, 228, 454, 455, 623, 624, 625, 1248, 1249, 5000, 10000):
         assert int(outs[k - 1]) == oracle.mt19937_nth(seed, k), k
+
+
+def _mt_stream(seed, count):
+    mt = np.zeros(624, dtype=np.uint32)
+    x = seed
+    mt[0] = x
+    for i in range(1, 624):
+        x = (1812433253 * (x ^ (x >> 30)) + i) & 0xFFFFFFFF
+        mt[i] = x
+    outs = []
+    while sum(o.shape[0] for o in outs) < count:
+        _twist_in_phases(mt)
+        outs.append(_temper(mt.copy()))
+    return np.concatenate(outs)[:count]
+
+
+# a non-symmetric CSR: 0 -> {1, 2}, 1 -> {0}, 2 -> {} (no neighbours),
+# 3 -> {0, 4}, 4 -> {3}
+SINK_IDX = np.array([0, 2, 3, 3, 5, 6], dtype=np.uint32)
+SINK_ADJ = np.array([1, 2, 0, 0, 4, 3], dtype=np.uint32)
+
+
+def sink_walks_brute_force(depth, n_walks, variant):
+    """The walk rule restated on the draws of each source's own stream: a
+    walker at a vertex without neighbours stops; its draws stay taken."""
+    n = SINK_IDX.shape[0] - 1
+    cnt = np.zeros(n, dtype=np.uint32)
+    for s in range(n):
+        if SINK_IDX[s + 1] == SINK_IDX[s]:
+            continue
+        draws = _mt_stream(42 + s, n_walks * depth)
+        for w in range(n_walks):
+            cur = s
+            for h in range(depth):
+                o = w * depth + h if variant == 0 else (w // 8) * 8 * depth + 8 * h + w % 8
+                d = int(SINK_IDX[cur + 1] - SINK_IDX[cur])
+                if d == 0:
+                    break
+                cur = int(SINK_ADJ[SINK_IDX[cur] + int(draws[o]) % d])
+                cnt[cur] += 1
+    return cnt
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_walker_stops_at_vertex_without_neighbours(variant):
+    exp = sink_walks_brute_force(7, 24, variant)
+    got = oracle.mt_walks(SINK_IDX, SINK_ADJ, 7, 24, variant=variant)
+    assert np.array_equal(got, exp)
+    assert exp[2] > 0  # some walkers did reach the vertex without neighbours

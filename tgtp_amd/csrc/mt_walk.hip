@@ -127,6 +127,10 @@ __global__ void __launch_bounds__(kMtBlock) mt_walk_kernel(MtWalkArgs a) {
                                  : static_cast<uint64_t>(w / 8u) * 8u * depth + 8u * h + (w % 8u);
           const uint32_t r = ring[o % kRing];
           const uint2 d = a.rowd[cur];
+          // a vertex without neighbours (only in a non-symmetric CSR; the
+          // reference divides by zero there): the walker stops, and since
+          // each hop's draw index is fixed, no other walk moves in the stream
+          if (d.y == 0) break;
           cur = a.adj[d.x + r % d.y];
           if (cur < kMtHot)
             atomicAdd(&hot[cur], 1u);
