@@ -46,7 +46,10 @@ extern "C" {
 typedef struct tgtp_ctx tgtp_ctx;       /* owns a HIP stream, events, scratch */
 typedef struct tgtp_dgraph tgtp_dgraph; /* device-resident CSR (HBM) */
 
-/* Host CSR view (the fields of fuser::graph_t, graph.h:13-15). */
+/* Host CSR view (the fields of fuser::graph_t, graph.h:13-15).  Rows sorted,
+ * ids < n.  The walk entry points take any such CSR; tgtp_taus, the
+ * tgtp_score family and the batch also need both directions of every edge,
+ * as graph.h's loader stores them (TGTP_E_ARG otherwise). */
 typedef struct tgtp_graph {
   uint64_t n;          /* vertices */
   uint64_t m;          /* CSR entries (both directions) */

@@ -150,3 +150,14 @@ def test_invalid_inputs_rejected(gpu_ctx):
     with pytest.raises(TgtpError, match="omega"):
         gpu_ctx.score(g2, lam[:1], X2[:, :1], 0.05, 10)
     assert MODE_FAITHFUL == 0
+    # a caller's CSR missing a reverse edge (the reference's loader always
+    # stores both; its lower_bound would run past the row), or with an
+    # unsorted row: refused before any kernel indexes by them
+    idx = np.array([0, 2, 4, 6, 7], dtype=np.uint32)
+    adj = np.array([1, 2, 0, 2, 0, 1, 1], dtype=np.uint32)  # 3 -> 1, but not 1 -> 3
+    X4 = np.ones((4, 3), dtype=np.float32)
+    with pytest.raises(TgtpError, match="both directions"):
+        gpu_ctx.score(G.Graph(idx, adj), lam, X4, 0.05, 10)
+    adj_u = np.array([2, 1, 0, 2, 0, 1], dtype=np.uint32)  # triangle, row 0 unsorted
+    with pytest.raises(TgtpError, match="sorted"):
+        gpu_ctx.score(G.Graph(idx[:4], adj_u), lam, X4[:3], 0.05, 10)

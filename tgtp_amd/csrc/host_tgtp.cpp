@@ -74,20 +74,24 @@ std::vector<double> power_table(const double* lam, int omega, uint32_t k_rows) {
   return t;
 }
 
-std::vector<int32_t> symmetrize_sources(const uint32_t* idx, const uint32_t* adj, uint64_t n,
-                                        uint64_t m) {
+int symmetrize_sources(const uint32_t* idx, const uint32_t* adj, uint64_t n, uint64_t m,
+                       std::vector<int32_t>& src) {
+  for (uint64_t u = 0; u < n; ++u)
+    for (uint64_t p = idx[u] + 1; p < idx[u + 1]; ++p)
+      if (adj[p - 1] > adj[p]) return 1;
   // aesc.h:538-545: rev_preds[first position of u in row v] = preds[pos(u,v)],
   // rows u ascending, positions ascending (a later duplicate overwrites).
-  std::vector<int32_t> src(m, -1);
+  src.assign(m, -1);
   for (uint64_t u = 0; u < n; ++u) {
     for (uint64_t p = idx[u]; p < idx[u + 1]; ++p) {
       const uint32_t v = adj[p];
-      const uint32_t* first =
-          std::lower_bound(adj + idx[v], adj + idx[v + 1], static_cast<uint32_t>(u));
+      const uint32_t* end = adj + idx[v + 1];
+      const uint32_t* first = std::lower_bound(adj + idx[v], end, static_cast<uint32_t>(u));
+      if (first == end || *first != u) return 2;
       src[first - adj] = static_cast<int32_t>(p);
     }
   }
-  return src;
+  return 0;
 }
 
 }  // namespace tgtp_host

@@ -17,8 +17,12 @@ int edge_tau(const uint32_t* idx, const float* X, int omega, const double* lam, 
 std::vector<double> power_table(const double* lam, int omega, uint32_t k_rows);
 
 // For each CSR position t, the position whose pred lands in rev_preds[t]
-// during symmetrization (-1 if none).
-std::vector<int32_t> symmetrize_sources(const uint32_t* idx, const uint32_t* adj, uint64_t n,
-                                        uint64_t m);
+// during symmetrization (-1 if none).  The reference's loader always stores
+// sorted rows holding both directions of every edge; a caller's CSR may
+// not, and then its lower_bound would find a wrong position or one past v's
+// row.  Returns 0, 1 for an unsorted row, 2 for an edge (u, v) without
+// (v, u).
+int symmetrize_sources(const uint32_t* idx, const uint32_t* adj, uint64_t n, uint64_t m,
+                       std::vector<int32_t>& src);
 
 }  // namespace tgtp_host

@@ -658,8 +658,14 @@ int tgtp_ensure_prep(tgtp_ctx* ctx, tgtp_dgraph* dg) {
     TGTP_HIP_CHECK(ctx, hipMemcpyAsync(P->h_adj.data(), dg->adj, dg->m * 4,
                                        hipMemcpyDeviceToHost, s));
   TGTP_HIP_CHECK(ctx, hipStreamSynchronize(s));
-  std::vector<int32_t> src =
-      tgtp_host::symmetrize_sources(P->h_idx.data(), P->h_adj.data(), dg->n, dg->m);
+  std::vector<int32_t> src;
+  const int shape = tgtp_host::symmetrize_sources(P->h_idx.data(), P->h_adj.data(), dg->n, dg->m, src);
+  if (shape) {
+    delete P;
+    return set_error(ctx, TGTP_E_ARG,
+                     shape == 1 ? "graph rows must be sorted"
+                                : "graph must store both directions of every edge (no reverse of some (u, v))");
+  }
   if (hipMalloc(&P->d_src_of, std::max<uint64_t>(dg->m, 1) * 4) != hipSuccess) {
     delete P;
     return set_error(ctx, TGTP_E_NOMEM, "device allocation: symmetrization map");
