@@ -106,6 +106,20 @@ def test_batch_rejects_mixed_inputs(gpu_ctx, small_batch):
         gpu_ctx.score_batch(graphs[:2], [eig[0], (lam[:16], X[:, :16])], 0.05, 10)
     with pytest.raises(TgtpError, match="epsilon"):
         gpu_ctx.score_batch(graphs[:2], eig[:2], 1.5, 10)
+    # an id past its own graph (a valid id of the union) is refused per graph
+    g0 = graphs[0]
+    adj = g0.adj.copy()
+    adj[-1] = g0.n
+    with pytest.raises(TgtpError, match="graph 0: neighbour id >= n"):
+        gpu_ctx.score_batch([G.Graph(g0.idx, adj), graphs[1]], eig[:2], 0.05, 10)
+    # a missing reverse edge is refused for the union as for one graph
+    c = G.cycle(12)
+    assert c.adj[c.idx[0]:c.idx[1]].tolist() == [1, 11]
+    adj = c.adj.copy()
+    adj[c.idx[0] + 1] = 10  # 0 -> 10, but 10 -/-> 0
+    e3 = (np.array([1.0, 0.5, 0.2]), np.ones((12, 3), dtype=np.float32))
+    with pytest.raises(TgtpError, match="both directions"):
+        gpu_ctx.score_batch([G.Graph(c.idx, adj), c], [e3, e3], 0.05, 10)
 
 
 def test_batch_on_two_streams_equals_per_graph(gpu_ctx, small_batch, monkeypatch):

@@ -818,6 +818,15 @@ extern "C" int tgtp_batch_create(tgtp_ctx* ctx, int32_t count, const tgtp_graph*
     if (!graphs[k].idx || graphs[k].n == 0 || !eigens[k].values || !eigens[k].vectors ||
         eigens[k].omega < 2 || static_cast<uint64_t>(eigens[k].omega) > graphs[k].n)
       return set_error(ctx, TGTP_E_ARG, "graph or eigenpairs missing / omega out of range");
+    // each graph on its own: the union's checks would not see an id that
+    // points into the next graph's vertices
+    const tgtp_graph& g = graphs[k];
+    if ((g.m && !g.adj) || g.idx[0] != 0 || g.idx[g.n] != g.m)
+      return set_error(ctx, TGTP_E_ARG, "graph " + std::to_string(k) +
+                                            ": null adjacency or idx[0] != 0 or idx[n] != m");
+    for (uint64_t p = 0; p < g.m; ++p)
+      if (g.adj[p] >= g.n)
+        return set_error(ctx, TGTP_E_ARG, "graph " + std::to_string(k) + ": neighbour id >= n");
     for (uint64_t v = 0; v < graphs[k].n; ++v)
       if (graphs[k].idx[v + 1] == graphs[k].idx[v])
         return set_error(ctx, TGTP_E_ARG,
