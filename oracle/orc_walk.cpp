@@ -55,12 +55,21 @@ extern "C" void orc_saba_walk_source(const uint32_t* idx, const uint32_t* adj,
       sec[b] = (static_cast<uint32_t>(ii) + static_cast<uint32_t>(b + 1)) * step;  // :75-78
       st[b] = sec[b];                                                                // :79
     }
+    // A walker at a vertex without neighbours (reachable only in a
+    // non-symmetric CSR, where the reference would divide by zero) stops:
+    // it no longer moves, counts, or joins its bouquet's zero check.  In a
+    // graph storing both directions every walker stays live.
+    bool live[kLanes];
     uint32_t hop = 0;
     for (; hop < depth; ++hop) {
       bool any_zero = false;  // :83-85, checked before every hashed hop
-      for (int b = 0; b < kLanes; ++b) any_zero |= (sec[b] == 0);
+      for (int b = 0; b < kLanes; ++b) {
+        live[b] = (hop == 0 || live[b]) && deg(idx, cur[b]) != 0;
+        any_zero |= live[b] && sec[b] == 0;
+      }
       if (any_zero) break;
       for (int b = 0; b < kLanes; ++b) {
+        if (!live[b]) continue;
         const uint32_t d = deg(idx, cur[b]);
         const uint32_t k = mulhi32(sec[b] ^ orc_fmix32(cur[b]), d);
         cur[b] = adj[idx[cur[b]] + k];
@@ -71,6 +80,8 @@ extern "C" void orc_saba_walk_source(const uint32_t* idx, const uint32_t* adj,
     // Whole-bouquet SplitMix32 fallback for the remaining hops (:117-143).
     for (; hop < depth; ++hop) {
       for (int b = 0; b < kLanes; ++b) {
+        live[b] = live[b] && deg(idx, cur[b]) != 0;
+        if (!live[b]) continue;
         const uint32_t r = orc_splitmix32(&st[b]);
         const uint32_t k = r % deg(idx, cur[b]);
         cur[b] = adj[idx[cur[b]] + k];

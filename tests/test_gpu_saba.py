@@ -146,3 +146,17 @@ def test_full_scale_chunglu_1m(gpu_ctx):
             assert np.array_equal(part, ref), f"slice [{lo},{hi})"
     finally:
         dg.free()
+
+
+@pytest.mark.parametrize("packed,hot", [("0", "32768"), ("1", "32768"), ("1", "0")])
+def test_walker_stops_at_vertex_without_neighbours(gpu_ctx, monkeypatch, packed, hot):
+    """A non-symmetric CSR through the C ABI: walkers that reach a vertex
+    without neighbours stop and leave their bouquet's zero check, as in the
+    oracle (itself pinned by a brute-force restatement,
+    tests/test_oracle_pinning.py), on both walk layouts and counter paths."""
+    from test_oracle_pinning import SABA_SINK_ADJ, SABA_SINK_IDX
+    monkeypatch.setenv("TGTP_WALK_PACKED", packed)
+    monkeypatch.setenv("TGTP_WALK_HOT", hot)
+    g = G.Graph(SABA_SINK_IDX, SABA_SINK_ADJ)
+    for depth, W_ in ((12, 16), (40, 64), (60, 256)):
+        _check(gpu_ctx, g, depth, W_, label=f"sink packed={packed} hot={hot}")

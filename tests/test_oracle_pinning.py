@@ -158,3 +158,78 @@ def test_exact_foster_and_cg_agree():
     pos = np.arange(0, g.m, 97)
     Rc = sampled_cg(g.idx, g.adj, pos)
     assert np.allclose(Rc, R[pos], rtol=1e-9, atol=1e-11)
+
+
+# a non-symmetric CSR: 0 -> {1, 2, 3, 4}, 1 -> {0, 5}, 2 -> {0, 5}, 3 -> {0},
+# 4 -> {} and 6 -> {} (no neighbours), 5 -> {1, 2, 6}
+SABA_SINK_IDX = np.array([0, 4, 6, 8, 9, 9, 12, 12], dtype=np.uint32)
+SABA_SINK_ADJ = np.array([1, 2, 3, 4, 0, 5, 0, 5, 0, 1, 2, 6], dtype=np.uint32)
+
+
+def _py_fmix32(h):
+    h ^= h >> 16
+    h = (h * 0x85EBCA6B) & 0xFFFFFFFF
+    h ^= h >> 13
+    h = (h * 0xC2B2AE35) & 0xFFFFFFFF
+    return h ^ (h >> 16)
+
+
+def _py_splitmix32(s):
+    s = (s + 0x9E3779B9) & 0xFFFFFFFF
+    z = s ^ (s >> 16)
+    z = (z * 0x21F0AAAD) & 0xFFFFFFFF
+    z ^= z >> 15
+    z = (z * 0x735A2D97) & 0xFFFFFFFF
+    return z ^ (z >> 15), s
+
+
+def saba_sink_brute_force(idx, adj, depth, n_walks):
+    """rw_experiment_saba.cpp:65-145 restated, with the rule for a walker
+    that reaches a vertex without neighbours: it stops moving and counting
+    and leaves its bouquet's zero check.  Returns (counts, bouquets that fell
+    back, walkers stopped, hops where only a stopped walker's secret was 0)."""
+    n = idx.shape[0] - 1
+    deg = lambda v: int(idx[v + 1] - idx[v])  # noqa: E731
+    cnt = np.zeros(n, dtype=np.uint32)
+    step = 0xFFFFFFFF // n_walks
+    fell = stopped = dead_zero = 0
+    for s in range(n):
+        if deg(s) == 0:
+            continue
+        for ii in range(0, n_walks, 8):
+            cur = [s] * 8
+            sec = [((ii + b + 1) * step) & 0xFFFFFFFF for b in range(8)]
+            st = list(sec)
+            live = [True] * 8
+            fb = False
+            for _ in range(depth):
+                live = [live[b] and deg(cur[b]) > 0 for b in range(8)]
+                if not fb and any(live[b] and sec[b] == 0 for b in range(8)):
+                    fb = True
+                    fell += 1
+                elif not fb and any(not live[b] and sec[b] == 0 for b in range(8)):
+                    dead_zero += 1
+                for b in range(8):
+                    if not live[b]:
+                        continue
+                    d = deg(cur[b])
+                    if not fb:
+                        k = ((sec[b] ^ _py_fmix32(cur[b])) * d) >> 32
+                        sec[b] = (sec[b] * d) & 0xFFFFFFFF
+                    else:
+                        r, st[b] = _py_splitmix32(st[b])
+                        k = r % d
+                    cur[b] = int(adj[idx[cur[b]] + k])
+                    cnt[cur[b]] += 1
+            stopped += sum(not x for x in live)
+    return cnt, fell, stopped, dead_zero
+
+
+@pytest.mark.parametrize("depth,n_walks,falls,dead_zero", [
+    (12, 16, False, False), (40, 64, True, False), (60, 256, True, True)])
+def test_saba_walker_stops_at_vertex_without_neighbours(depth, n_walks, falls, dead_zero):
+    exp, fell, stopped, dz = saba_sink_brute_force(SABA_SINK_IDX, SABA_SINK_ADJ, depth, n_walks)
+    got = oracle.saba_walks(SABA_SINK_IDX, SABA_SINK_ADJ, depth, n_walks, threads=1)
+    assert np.array_equal(got, exp)
+    # the rules each case exercises
+    assert stopped > 0 and (fell > 0) == falls and (dz > 0) == dead_zero
