@@ -149,6 +149,8 @@ def test_invalid_inputs_rejected(gpu_ctx):
         gpu_ctx.score(g2, lam, X2, 1.5, 10)
     with pytest.raises(TgtpError, match="omega"):
         gpu_ctx.score(g2, lam[:1], X2[:, :1], 0.05, 10)
+    with pytest.raises(TgtpError, match="unknown tgtp mode"):
+        gpu_ctx.taus(g2, lam, X2, 0.025, 7)
     assert MODE_FAITHFUL == 0
     # a caller's CSR missing a reverse edge (the reference's loader always
     # stores both; its lower_bound would run past the row), or with an

@@ -973,8 +973,8 @@ extern "C" int tgtp_taus(tgtp_ctx* ctx, const tgtp_graph* g, const tgtp_eigens* 
                          float eps_half, int32_t mode, int32_t* taus_out, tgtp_stats* stats) {
   if (!ctx) return TGTP_E_ARG;
   if (!taus_out) return set_error(ctx, TGTP_E_ARG, "null output");
-
-
+  if (mode != TGTP_MODE_FAITHFUL && mode != TGTP_MODE_FAST_FP32 && mode != TGTP_MODE_FAST_BF16)
+    return set_error(ctx, TGTP_E_ARG, "unknown tgtp mode");
   if (!(eps_half > 0)) return set_error(ctx, TGTP_E_ARG, "epsilon must be positive");
   const double t0 = now_ms();
   tgtp_dgraph* dg = nullptr;
