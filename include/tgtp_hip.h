@@ -155,9 +155,9 @@ int tgtp_score(tgtp_ctx* ctx, const tgtp_graph* g, const tgtp_eigens* e,
                const tgtp_params* p, float* scores_out, tgtp_stats* stats);
 
 /* Same, on a graph already resident in HBM (eigenpairs are uploaded per call
- * unless the same vectors pointer and omega were passed on the previous call
- * with this dgraph; contents are not compared, so eigenpairs changed in
- * place need a fresh upload of the dgraph or another buffer). */
+ * unless the same vectors and values pointers and omega were passed on the
+ * previous call with this dgraph; contents are not compared, so eigenpairs
+ * changed in place need a fresh upload of the dgraph or other buffers). */
 int tgtp_score_dev(tgtp_ctx* ctx, const tgtp_dgraph* dg, const tgtp_eigens* e,
                    const tgtp_params* p, float* scores_out, tgtp_stats* stats);
 

@@ -197,3 +197,25 @@ def test_fast_walk_sums_independent_of_small_bq(gpu_ctx, monkeypatch):
         out.append(gpu_ctx.score(g, lam, X, 0.01, 10, mode=MODE_FAST_FP32)[0])
     for s in out[1:]:
         assert np.array_equal(out[0].view(np.uint32), s.view(np.uint32))
+
+
+@pytest.mark.parametrize("mode", [MODE_FAITHFUL, MODE_FAST_FP32])
+def test_device_graph_eigenpair_cache_keys(gpu_ctx, mode):
+    """tgtp_score_dev re-uploads the eigenpairs (and rebuilds a FAST plan)
+    when either the vectors or the values pointer changes: new eigenvalues
+    with the same vectors give the scores of a fresh device graph."""
+    g = G.chung_lu(600, 2.5, 9.0, seed=31)
+    lam, X = E.eigens_f32(*E.compute_eigens(g, 16))
+    lam2 = lam.copy()
+    lam2[1:] *= 0.9  # other eigenvalues, same vectors array
+    dg = gpu_ctx.upload(g)
+    try:
+        s1, _ = gpu_ctx.score(dg, lam, X, 0.05, 10, mode=mode)
+        s2, _ = gpu_ctx.score(dg, lam2, X, 0.05, 10, mode=mode)
+        s3, _ = gpu_ctx.score(dg, lam, X, 0.05, 10, mode=mode)
+    finally:
+        dg.free()
+    f2, _ = gpu_ctx.score(g, lam2, X, 0.05, 10, mode=mode)
+    assert s1.tobytes() == s3.tobytes()
+    assert s2.tobytes() == f2.tobytes()
+    assert s1.tobytes() != s2.tobytes()

@@ -53,9 +53,12 @@ struct tgtp_dgraph {
   // tgtp host-side state (host CSR copy, symmetrization map), built lazily
   void* prep = nullptr;
   void (*prep_free)(void*) = nullptr;
-  // cached eigenpairs
+  // cached eigenpairs: the caller's vectors / values pointers and omega of
+  // the last upload, and a count of uploads (plans built from them match it)
   const void* eig_key = nullptr;
+  const void* eig_vals = nullptr;
   int32_t eig_omega = 0;
+  uint64_t eig_gen = 0;
   double* lam = nullptr;    // [omega]
   float* X = nullptr;       // [n*omega]
 };

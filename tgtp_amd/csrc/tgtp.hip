@@ -685,7 +685,12 @@ int tgtp_ensure_prep(tgtp_ctx* ctx, tgtp_dgraph* dg) {
 int tgtp_ensure_eigens(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e) {
   if (!e || !e->values || !e->vectors || e->omega < 2 || static_cast<uint64_t>(e->omega) > dg->n)
     return set_error(ctx, TGTP_E_ARG, "omega must be between 2 and vertex count");
-  if (dg->eig_key == e->vectors && dg->eig_omega == e->omega && dg->X) return TGTP_OK;
+  if (dg->eig_key == e->vectors && dg->eig_vals == e->values && dg->eig_omega == e->omega &&
+      dg->X)
+    return TGTP_OK;
+  // forget the old key first: a failed upload must not leave it matching
+  dg->eig_key = nullptr;
+  dg->eig_vals = nullptr;
   if (dg->X) (void)hipFree(dg->X);
   if (dg->lam) (void)hipFree(dg->lam);
   dg->X = nullptr;
@@ -698,7 +703,9 @@ int tgtp_ensure_eigens(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e) {
                                      ctx->stream));
   TGTP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   dg->eig_key = e->vectors;
+  dg->eig_vals = e->values;
   dg->eig_omega = e->omega;
+  ++dg->eig_gen;
   return TGTP_OK;
 }
 

@@ -54,7 +54,7 @@ struct GraphSpec {        // host description of one graph of a plan
 
 struct FastPlan {
   // identity (a plan cached on a single dgraph is reused when these match)
-  const void* eig_key = nullptr;
+  uint64_t eig_gen = ~0ull;  // eigenpair upload of the graph the plan was built for
   double eps = 0, delta = 0;
   int32_t gamma = 0, mode = 0;
   uint32_t n_graphs = 0;
@@ -181,7 +181,7 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
     P->eps = specs[0].eps;
     P->delta = specs[0].delta;
     P->gamma = specs[0].gamma;
-    P->eig_key = dg->eig_key;
+    P->eig_gen = dg->eig_gen;
   }
   // per-graph scalars
   std::vector<uint32_t> voff(G + 1);
@@ -660,7 +660,7 @@ int check_status(tgtp_ctx* ctx, FastPlan* P) {
 }
 
 bool plan_matches(const FastPlan* P, const tgtp_dgraph* dg, const tgtp_params* p) {
-  return P && P->eig_key == dg->eig_key && P->eps == p->epsilon && P->delta == p->delta &&
+  return P && P->eig_gen == dg->eig_gen && P->eps == p->epsilon && P->delta == p->delta &&
          P->gamma == p->gamma && P->mode == p->mode && P->omega == dg->eig_omega;
 }
 
