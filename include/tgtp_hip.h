@@ -171,7 +171,9 @@ int tgtp_score_dev(tgtp_ctx* ctx, const tgtp_dgraph* dg, const tgtp_eigens* e,
  * share one mode; FAST batches need one omega in [3, 128].  FAITHFUL batches
  * run the graphs one after another (bit-exact as tgtp_score).  Per-graph
  * scores equal what tgtp_score_dev returns for that graph with the same
- * tuning knobs.  scores_out[k] holds graphs[k].m floats. */
+ * tuning knobs.  scores_out[k] holds graphs[k].m floats.  FAST batches copy
+ * every input at create; FAITHFUL batches copy the graphs but read the
+ * eigenpair arrays when they run, so those must outlive the batch. */
 typedef struct tgtp_batch tgtp_batch;
 #define TGTP_BATCH_AUTOTUNE 1 /* time candidate launch shapes once, keep the best */
 #define TGTP_BATCH_GRAPH 2    /* capture the run into a hipGraph, replay it */
