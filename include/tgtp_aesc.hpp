@@ -74,13 +74,13 @@ struct AESCGraph {
 
   AESCGraph(const std::string& graph_path, const std::string& eigen_path, size_t omega) {
     char err[512] = {0};
-    int rc = tgtp_load_txt(graph_path.c_str(), &csr_, err, sizeof(err));
+    int rc = tgtp_load_txt(graph_path.c_str(), &csr_.c, err, sizeof(err));
     if (rc) throw std::runtime_error(err);
-    n = csr_.n;
-    m = csr_.m;
-    idx = csr_.idx;
-    adj = csr_.adj;
-    if (tgtp_check_aesc_graph(&csr_, static_cast<int32_t>(omega), err, sizeof(err)))
+    n = csr_.c.n;
+    m = csr_.c.m;
+    idx = csr_.c.idx;
+    adj = csr_.c.adj;
+    if (tgtp_check_aesc_graph(&csr_.c, static_cast<int32_t>(omega), err, sizeof(err)))
       throw std::runtime_error(err);
     eigens.len = n;
     eigens.omega = omega;
@@ -94,14 +94,21 @@ struct AESCGraph {
   }
   AESCGraph(const AESCGraph&) = delete;
   AESCGraph& operator=(const AESCGraph&) = delete;
-  ~AESCGraph() { tgtp_csr_free(&csr_); }
   uint32_t degree(uint32_t v) const { return idx[v + 1] - idx[v]; }
   uint32_t* begin(uint32_t v) { return adj + idx[v]; }
   uint32_t* end(uint32_t v) { return adj + idx[v + 1]; }
   void free() {}  // storage is released by the destructor
 
  private:
-  tgtp_csr csr_{};
+  // owns the loaded CSR; a member, so that it is also released when the
+  // constructor throws after loading the graph
+  struct Owned {
+    tgtp_csr c{};
+    Owned() = default;
+    Owned(const Owned&) = delete;
+    Owned& operator=(const Owned&) = delete;
+    ~Owned() { tgtp_csr_free(&c); }
+  } csr_;
 };
 
 struct AESCConfig {

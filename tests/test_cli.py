@@ -76,6 +76,55 @@ def test_tgtp_cli_scores_bin_matches_oracle(tmp_path):
     assert np.array_equal(s.view(np.uint32), ref.view(np.uint32))
 
 
+def test_aesc_dropin_header_errors_release_memory(tmp_path):
+    """The example built on include/tgtp_aesc.hpp, compiled here with
+    AddressSanitizer: AESCGraph throws the reference's messages for a graph
+    with an isolated vertex and for a missing eigenpair file, and releases
+    the graph it had loaded (no leak report).  No GPU is touched: the
+    constructor fails before any device call."""
+    import shutil
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("g++ not available")
+    root = BIN.parents[1]
+    exe = tmp_path / "aesc_asan"
+    build = subprocess.run(
+        [cxx, "-std=c++20", "-O1", "-g", "-fsanitize=address", f"-I{root / 'include'}",
+         str(root / "tgtp_amd" / "csrc" / "aesc_example.cpp"), "-o", str(exe),
+         f"-L{root / 'tgtp_amd'}", "-ltgtp_hip", f"-Wl,-rpath,{root / 'tgtp_amd'}", "-fopenmp"],
+        capture_output=True, text=True)
+    if build.returncode != 0 and "asan" in build.stderr.lower():
+        pytest.skip("sanitizer runtime not available")
+    assert build.returncode == 0, build.stderr[-2000:]
+    iso = tmp_path / "iso"
+    iso.mkdir()
+    (iso / "graph.txt").write_text("0 1\n1 2\n2 0\n4 5\n")  # vertex 3 isolated
+    (iso / "sorted_eigens_2.txt").write_text("1 1 1 1 1 1 1\n0.5 1 1 1 1 1 1\n")
+    noeig = tmp_path / "noeig"
+    noeig.mkdir()
+    (noeig / "graph.txt").write_text("0 1\n1 2\n2 0\n")
+    for name, msg in (("iso", "no isolated vertices"), ("noeig", "cannot open eigenpairs")):
+        r = run([exe, tmp_path, name, "0.05", "2", "10", tmp_path / "out.bin"])
+        assert r.returncode == 1 and msg in r.stderr, r.stderr
+        assert "LeakSanitizer" not in r.stderr, r.stderr
+
+
+@pytest.mark.gpu
+def test_aesc_dropin_header_matches_cli_and_oracle(tmp_path):
+    """AESC(graph, config).tgtp() through include/tgtp_aesc.hpp gives the
+    bits of the tgtp binary and of the oracle."""
+    g = G.chung_lu(700, 2.5, 10.0, seed=16)
+    lam, X = _dataset(tmp_path, "cl700", g, 24)
+    a, b = tmp_path / "a.bin", tmp_path / "b.bin"
+    r = run([BIN / "aesc_example", tmp_path, "cl700", "0.05", "24", "10", a])
+    assert r.returncode == 0, r.stderr
+    r = run([BIN / "tgtp", tmp_path, "cl700", "0.05", "24", "10", b])
+    assert r.returncode == 0, r.stderr
+    assert a.read_bytes() == b.read_bytes()
+    ref = oracle.tgtp(g.idx, g.adj, lam, X, 0.05, 10)
+    assert np.array_equal(np.fromfile(a, dtype=np.float32).view(np.uint32), ref.view(np.uint32))
+
+
 @pytest.mark.gpu
 def test_tgtp_cli_binary_caches(tmp_path):
     """TGTP_CACHE=1: the first run writes graph.txt.csr and

@@ -43,7 +43,8 @@ HOST_SOURCES = ["graph_io.cpp", "host_tgtp.cpp"]
 CLI_SOURCES = {"rw_experiment_saba": ("cli_rw.cpp", []),
                "rw_experiment": ("cli_rw.cpp", ["-DTGTP_RW_VARIANT=1"]),
                "rw_experiment_simd": ("cli_rw.cpp", ["-DTGTP_RW_VARIANT=2"]),
-               "tgtp": ("cli_tgtp.cpp", []), "benchmark": ("cli_benchmark.cpp", [])}
+               "tgtp": ("cli_tgtp.cpp", []), "benchmark": ("cli_benchmark.cpp", []),
+               "aesc_example": ("aesc_example.cpp", [])}
 
 
 def _run(cmd: list[str]) -> None:
