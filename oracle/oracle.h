@@ -19,7 +19,7 @@
  *       SplitMix32 fallback) plus closed-form invariants.
  *   orc_load_txt / orc_load_eigens / orc_taus / orc_tgtp (walk mode MT19937):
  *       bit-exact against the reference's own aesc_baseline.h and graph.h,
- *       compiled from /root/reference by oracle/Makefile (target ref).
+ *       compiled from the reference tree by oracle/Makefile (target ref).
  *   orc_mt_walks variant 0 (regular): per-vertex counts equal the reference's
  *       own scalar_randomwalk (rw_experiment.cpp, compiled by oracle/Makefile,
  *       target ref); the generator is std::mt19937, checked by the standard's
@@ -28,6 +28,11 @@
  *       Intel SVML (_mm256_rem_epu32), so it is not built; at depth 1 both
  *       variants draw the same outputs and must agree (tested).  The hop
  *       order of the draws at depth > 1 is PARITY UNPINNED.
+ *   orc_saba_walks / orc_mt_walks on a CSR without both directions of every
+ *       edge (the reference divides by zero there): a walker at a vertex
+ *       without neighbours stops; checked against brute-force Python
+ *       restatements of that rule (tests/test_oracle_pinning.py,
+ *       tests/test_mt_walks.py).
  *   orc_tgtp (walk mode SABA): everything except the 2-way walk is the code
  *       path pinned above.  Phase 1 of the 2-way walk is checked against the
  *       pinned 1-way walk (same grid, hash and multiply-high step).
