@@ -4,7 +4,8 @@
 // of aesc.h:538-545 -- rev_preds[first position of u in row v] takes
 // preds[pos(u, v)], rows u ascending, positions ascending, the last write
 // winning -- and the return code is 0; an unsorted row gives 1, an edge
-// without its reverse 2.
+// without its reverse 2 -- on small graphs and on graphs large enough for the
+// host threads to split the rows.
 // usage: symmetrize_check <cases>; prints "ok <cases>" on success.
 #include <algorithm>
 #include <cstdio>
@@ -104,6 +105,25 @@ int main(int argc, char** argv) {
         }
         break;
       }
+  }
+  // graphs large enough that the host threads split the rows
+  for (int c = 0; c < 3; ++c) {
+    const uint32_t n = 30000 + static_cast<uint32_t>(rng() % 20000);
+    std::vector<std::pair<uint32_t, uint32_t>> pairs;
+    for (uint32_t k = 0; k < 6 * n; ++k) {
+      const uint32_t a = static_cast<uint32_t>(rng() % n);
+      const uint32_t b = rng() % 4 == 0 ? static_cast<uint32_t>(rng() % 64)  // hubs
+                                        : static_cast<uint32_t>(rng() % n);
+      pairs.emplace_back(a, b);
+      if (rng() % 7 == 0) pairs.emplace_back(b, a);  // duplicate
+    }
+    const Csr g = from_pairs(n, pairs);
+    std::vector<int32_t> src;
+    const int rc = tgtp_host::symmetrize_sources(g.idx.data(), g.adj.data(), n, g.adj.size(), src);
+    if (rc != 0 || src != direct(g)) {
+      std::printf("large case %d: rc %d or map differs\n", c, rc);
+      return 1;
+    }
   }
   std::printf("ok %d\n", cases);
   return 0;

@@ -76,22 +76,36 @@ std::vector<double> power_table(const double* lam, int omega, uint32_t k_rows) {
 
 int symmetrize_sources(const uint32_t* idx, const uint32_t* adj, uint64_t n, uint64_t m,
                        std::vector<int32_t>& src) {
-  for (uint64_t u = 0; u < n; ++u)
+  const int64_t rows = static_cast<int64_t>(n);
+  int unsorted = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : unsorted)
+  for (int64_t u = 0; u < rows; ++u)
     for (uint64_t p = idx[u] + 1; p < idx[u + 1]; ++p)
-      if (adj[p - 1] > adj[p]) return 1;
+      if (adj[p - 1] > adj[p]) {
+        ++unsorted;
+        break;
+      }
+  if (unsorted) return 1;
   // aesc.h:538-545: rev_preds[first position of u in row v] = preds[pos(u,v)],
   // rows u ascending, positions ascending (a later duplicate overwrites).
+  // Slot (first u in row v) is written only from row u, so rows run in
+  // parallel and each row in position order: the same map.
   src.assign(m, -1);
-  for (uint64_t u = 0; u < n; ++u) {
+  int missing = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : missing)
+  for (int64_t u = 0; u < rows; ++u) {
     for (uint64_t p = idx[u]; p < idx[u + 1]; ++p) {
       const uint32_t v = adj[p];
       const uint32_t* end = adj + idx[v + 1];
       const uint32_t* first = std::lower_bound(adj + idx[v], end, static_cast<uint32_t>(u));
-      if (first == end || *first != u) return 2;
+      if (first == end || *first != u) {
+        ++missing;
+        break;
+      }
       src[first - adj] = static_cast<int32_t>(p);
     }
   }
-  return 0;
+  return missing ? 2 : 0;
 }
 
 }  // namespace tgtp_host
