@@ -37,8 +37,11 @@
  *       path pinned above.  Phase 1 of the 2-way walk is checked against the
  *       pinned 1-way walk (same grid, hash and multiply-high step).
  *       The fallback phase of the 2-way walk (two successive SplitMix32 draws
- *       from one state, aesc.h:339-340) is PARITY UNPINNED: the only
- *       reference code that runs it needs Intel SVML, which this image lacks.
+ *       from one state, aesc.h:339-340): that both walkers share one state is
+ *       checked by an identical-endpoints property (they part only after the
+ *       fallback); which walker takes the first draw is PARITY UNPINNED: the
+ *       only reference code that runs it needs Intel SVML, which this image
+ *       lacks.
  */
 #ifndef TGTP_ORACLE_H
 #define TGTP_ORACLE_H

@@ -112,6 +112,22 @@ def test_walk2_identical_endpoints_cancel_before_fallback():
     assert oracle.walk2_saba(g.idx, g.adj, 4, 4, 32, 12, pvec) == 0.0
 
 
+def test_walk2_fallback_draws_both_walkers_from_one_state():
+    """aesc.h:339-340: in the fallback both walkers draw from one SplitMix32
+    state, one after the other.  From identical endpoints the hashed phase
+    keeps the two walkers together (x stays 0: on C9 every secret reaches 0
+    within 32 hops, before which the sum is exactly 0); once the bouquet
+    falls back they take different draws and part, so x != 0.  Two equal
+    copies of the state (a plausible misreading) would keep x == 0.  Which
+    walker takes the first of the two draws stays unpinned: swapping them
+    only negates x here."""
+    g = G.cycle(9)
+    pvec = np.linspace(0.1, 1.0, g.n, dtype=np.float32)
+    for nr, L in [(64, 60), (16, 40), (8, 33)]:
+        assert oracle.walk2_saba(g.idx, g.adj, 4, 4, nr, 20, pvec) == 0.0
+        assert oracle.walk2_saba(g.idx, g.adj, 4, 4, nr, L, pvec) != 0.0
+
+
 def test_pow_square_compiles_to_product():
     """aesc.h:498 squares with std::pow(x, 2).  g++ folds that to x*x (the
     correctly rounded square), so an IEEE build of the reference -- and this
