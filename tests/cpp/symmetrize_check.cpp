@@ -106,18 +106,23 @@ int main(int argc, char** argv) {
         break;
       }
   }
-  // graphs large enough that the host threads split the rows
-  for (int c = 0; c < 3; ++c) {
-    const uint32_t n = 30000 + static_cast<uint32_t>(rng() % 20000);
+  // graphs large enough (over 2^20 CSR entries) that the host threads split
+  // the rows
+  for (int c = 0; c < 2; ++c) {
+    const uint32_t n = 100000 + static_cast<uint32_t>(rng() % 20000);
     std::vector<std::pair<uint32_t, uint32_t>> pairs;
     for (uint32_t k = 0; k < 6 * n; ++k) {
       const uint32_t a = static_cast<uint32_t>(rng() % n);
-      const uint32_t b = rng() % 4 == 0 ? static_cast<uint32_t>(rng() % 64)  // hubs
-                                        : static_cast<uint32_t>(rng() % n);
+      const uint32_t b = rng() % 64 == 0 ? static_cast<uint32_t>(rng() % 64)  // hubs
+                                         : static_cast<uint32_t>(rng() % n);
       pairs.emplace_back(a, b);
       if (rng() % 7 == 0) pairs.emplace_back(b, a);  // duplicate
     }
     const Csr g = from_pairs(n, pairs);
+    if (g.adj.size() < (1u << 20)) {
+      std::printf("large case %d: only %zu entries\n", c, g.adj.size());
+      return 1;
+    }
     std::vector<int32_t> src;
     const int rc = tgtp_host::symmetrize_sources(g.idx.data(), g.adj.data(), n, g.adj.size(), src);
     if (rc != 0 || src != direct(g)) {

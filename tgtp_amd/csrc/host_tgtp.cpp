@@ -77,8 +77,12 @@ std::vector<double> power_table(const double* lam, int omega, uint32_t k_rows) {
 int symmetrize_sources(const uint32_t* idx, const uint32_t* adj, uint64_t n, uint64_t m,
                        std::vector<int32_t>& src) {
   const int64_t rows = static_cast<int64_t>(n);
+  // host threads only for large graphs: on small ones waking them (and their
+  // spinning afterwards) costs more than the loop, and delays the launches
+  // that follow
+  const bool par = m >= (1ull << 20);
   int unsorted = 0;
-#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : unsorted)
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : unsorted) if (par)
   for (int64_t u = 0; u < rows; ++u)
     for (uint64_t p = idx[u] + 1; p < idx[u + 1]; ++p)
       if (adj[p - 1] > adj[p]) {
@@ -92,7 +96,7 @@ int symmetrize_sources(const uint32_t* idx, const uint32_t* adj, uint64_t n, uin
   // parallel and each row in position order: the same map.
   src.assign(m, -1);
   int missing = 0;
-#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : missing)
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : missing) if (par)
   for (int64_t u = 0; u < rows; ++u) {
     for (uint64_t p = idx[u]; p < idx[u + 1]; ++p) {
       const uint32_t v = adj[p];
