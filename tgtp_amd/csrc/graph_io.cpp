@@ -190,7 +190,9 @@ int build_csr_chunks(uint64_t n, const std::vector<ParseChunk>& chunks, tgtp_csr
   }
   std::vector<uint32_t> cnt(n, 0);
   const int nc = static_cast<int>(chunks.size());
-#pragma omp parallel for schedule(static, 1)
+  // (host threads are woken only when there is work for several: idle ones
+  // spin for a while after a parallel loop and slow what follows)
+#pragma omp parallel for schedule(static, 1) if (nc > 1)
   for (int t = 0; t < nc; ++t)
     for (const auto& e : chunks[static_cast<size_t>(t)].edges) {
       __atomic_fetch_add(&cnt[e.first], 1u, __ATOMIC_RELAXED);
@@ -199,13 +201,13 @@ int build_csr_chunks(uint64_t n, const std::vector<ParseChunk>& chunks, tgtp_csr
   idx[0] = 0;
   for (uint64_t v = 0; v < n; ++v) idx[v + 1] = idx[v] + cnt[v];
   std::copy(idx, idx + n, cnt.begin());  // next free slot per row
-#pragma omp parallel for schedule(static, 1)
+#pragma omp parallel for schedule(static, 1) if (nc > 1)
   for (int t = 0; t < nc; ++t)
     for (const auto& e : chunks[static_cast<size_t>(t)].edges) {
       adj[__atomic_fetch_add(&cnt[e.first], 1u, __ATOMIC_RELAXED)] = e.second;
       adj[__atomic_fetch_add(&cnt[e.second], 1u, __ATOMIC_RELAXED)] = e.first;
     }
-#pragma omp parallel for schedule(dynamic, 4096)
+#pragma omp parallel for schedule(dynamic, 4096) if (m >= (1ull << 20))
   for (int64_t v = 0; v < static_cast<int64_t>(n); ++v) std::sort(adj + idx[v], adj + idx[v + 1]);
   out->n = n;
   out->m = m;
@@ -263,7 +265,7 @@ int build_csr(uint64_t n, const std::vector<std::pair<uint32_t, uint32_t>>& edge
     adj[fill[e.first]++] = e.second;
     adj[fill[e.second]++] = e.first;
   }
-#pragma omp parallel for schedule(dynamic, 4096)
+#pragma omp parallel for schedule(dynamic, 4096) if (m >= (1ull << 20))
   for (int64_t v = 0; v < static_cast<int64_t>(n); ++v) std::sort(adj + idx[v], adj + idx[v + 1]);
   out->n = n;
   out->m = m;
@@ -473,7 +475,7 @@ extern "C" int tgtp_load_csr_bin(const char* path, tgtp_csr* out, char* err, siz
   bool ok = idx[0] == 0 && idx[n] == m;
   for (uint64_t v = 0; ok && v < n; ++v) ok = idx[v] <= idx[v + 1];
   int bad_rows = 0;
-#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : bad_rows)
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : bad_rows) if (m >= (1ull << 20))
   for (int64_t v = 0; v < (ok ? static_cast<int64_t>(n) : 0); ++v)
     for (uint32_t p = idx[v]; p < idx[v + 1]; ++p)
       if (adj[p] >= n || (p > idx[v] && adj[p - 1] > adj[p])) {
@@ -500,7 +502,7 @@ extern "C" int tgtp_save_eigens_bin(const char* path, uint64_t n, int32_t omega,
   for (uint64_t i = 0; i < om; ++i)
     if (!std::isfinite(values[i])) return -1;
   int nonfinite = 0;
-#pragma omp parallel for schedule(static) reduction(+ : nonfinite)
+#pragma omp parallel for schedule(static) reduction(+ : nonfinite) if (cnt >= (1ull << 20))
   for (int64_t i = 0; i < static_cast<int64_t>(cnt); ++i) nonfinite += !std::isfinite(vectors[i]);
   if (nonfinite) return -1;
   const std::string tmp = std::string(path) + ".tmp";
@@ -560,7 +562,7 @@ extern "C" int tgtp_load_eigens_bin(const char* path, uint64_t n, int32_t omega,
   bool finite = true;
   for (uint64_t i = 0; i < om; ++i) finite = finite && std::isfinite(values[i]);
   int nonfinite = 0;
-#pragma omp parallel for schedule(static) reduction(+ : nonfinite)
+#pragma omp parallel for schedule(static) reduction(+ : nonfinite) if (n * om >= (1ull << 20))
   for (int64_t i = 0; i < static_cast<int64_t>(n * om); ++i) nonfinite += !std::isfinite(vectors[i]);
   if (!finite || nonfinite) {
     set_err(err, errlen, bad + " (non-finite values)");
@@ -806,7 +808,7 @@ extern "C" int tgtp_write_graph_txt(const tgtp_csr* g, const char* path) {
 }
 
 extern "C" int tgtp_csr_reverse_index(const tgtp_csr* g, uint32_t* rev) {
-#pragma omp parallel for schedule(dynamic, 1024)
+#pragma omp parallel for schedule(dynamic, 1024) if (g->m >= (1ull << 20))
   for (int64_t u = 0; u < static_cast<int64_t>(g->n); ++u) {
     for (uint64_t p = g->idx[u]; p < g->idx[u + 1]; ++p) {
       const uint32_t v = g->adj[p];
