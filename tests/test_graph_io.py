@@ -1,5 +1,7 @@
 """Host graph I/O (graph.h / aesc.h loader semantics) and the seeded
 generators.  CPU only."""
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -336,3 +338,46 @@ def test_load_txt_cached(tmp_path):
     g3 = G.load_txt_cached(p)  # damaged cache: parsed again and rewritten
     assert np.array_equal(g1.adj, g3.adj)
     assert G.load_bin(cache).m == g1.m
+
+
+_THREADS_PROBE = r"""
+import hashlib, sys
+sys.path.insert(0, sys.argv[1])
+import numpy as np
+from tgtp_amd import graph as G
+h = hashlib.sha256()
+g = G.chung_lu(200000, 2.3, 12.0, seed=9)   # above the generator's parallel thresholds
+h.update(g.idx.tobytes()); h.update(g.adj.tobytes())
+g2 = G.grid_rewired(300, 300, 0.05, seed=4)
+h.update(g2.idx.tobytes()); h.update(g2.adj.tobytes())
+t = G.load_txt(sys.argv[2])
+h.update(t.idx.tobytes()); h.update(t.adj.tobytes())
+lam, X = G.load_eigens_txt(sys.argv[3], int(sys.argv[4]), int(sys.argv[5]))
+h.update(lam.tobytes()); h.update(X.tobytes())
+print(h.hexdigest())
+"""
+
+
+def test_host_results_independent_of_thread_count(tmp_path):
+    """The generators, the graph loader and the eigenpair loader give the
+    same bits on 1 and on 8 host threads (inputs large enough for every
+    parallel path)."""
+    import os
+    import subprocess
+    import sys
+    root = str(Path(__file__).resolve().parents[1])
+    g = G.chung_lu(150000, 2.5, 10.0, seed=3)
+    gp = tmp_path / "g.txt"
+    g.write_txt(gp)  # several MiB: parsed in chunks
+    n, om = 40000, 8
+    rng = np.random.default_rng(2)
+    ep = tmp_path / "e.txt"
+    E.write_eigens_txt(ep, rng.standard_normal(om), rng.standard_normal((n, om)).astype(np.float32))
+    digests = []
+    for threads in ("1", "8"):
+        env = dict(os.environ, OMP_NUM_THREADS=threads)
+        r = subprocess.run([sys.executable, "-c", _THREADS_PROBE, root, str(gp), str(ep), str(n),
+                            str(om)], capture_output=True, text=True, env=env, timeout=600)
+        assert r.returncode == 0, r.stderr[-2000:]
+        digests.append(r.stdout.strip())
+    assert digests[0] == digests[1]
