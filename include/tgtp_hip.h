@@ -76,6 +76,18 @@ enum {
   TGTP_MODE_FAST_BF16 = 2
 };
 
+/* Tau refinement rule of the FAST modes (a flag OR'ed into the mode).
+ * The reference refines tau with get_tau(..., delta, upsilon), where delta is
+ * the top-omega tail of the series after t steps, and when delta >= eps/2 it
+ * bounds the remaining terms against eps/2 instead of eps/2 - delta
+ * (aesc.h:130): the tail is then left out of the bound and tau can drop to 1
+ * on an edge whose tail alone exceeds eps/2 (bridges into sparse parts of a
+ * graph; errors up to 0.5 on the batched workload's graphs).  By default the
+ * FAST modes skip such a refinement step (t advances, tau is kept), so every
+ * tau satisfies  delta(tau) + rest(tau) <= eps/2.  TGTP_TAU_REFERENCE selects
+ * the reference's rule in FAST modes; FAITHFUL always uses it. */
+#define TGTP_TAU_REFERENCE 0x100
+
 /* AESCConfig (aesc.h:98-102); the reference CLI sets delta = 1/n
  * (main.cpp:36).  gamma >= 1, 0 < epsilon < 1. */
 typedef struct tgtp_params {

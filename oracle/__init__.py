@@ -60,6 +60,9 @@ def lib() -> C.CDLL:
         L.orc_tgtp_rows.argtypes = [C.c_uint64, C.c_uint64, u32p, u32p, P(C.c_int32), C.c_double,
                                     C.c_double, C.c_int, C.c_uint32, C.c_uint32, C.c_int,
                                     P(C.c_float)]
+        L.orc_tgtp_sources.argtypes = [C.c_uint64, C.c_uint64, u32p, u32p, P(C.c_int32),
+                                       C.c_double, C.c_double, C.c_int, u32p, C.c_uint64,
+                                       C.c_int, P(C.c_float)]
         L.orc_load_txt.argtypes = [C.c_char_p, P(C.c_uint64), P(C.c_uint64), P(u32p), P(u32p)]
         L.orc_load_eigens.argtypes = [C.c_char_p, C.c_uint64, C.c_int, P(C.c_double),
                                       P(C.c_float)]
@@ -185,6 +188,25 @@ def tgtp_rows(idx, adj, taus_arr, epsilon: float, gamma: int, src_begin: int, sr
     out = np.zeros(m, dtype=np.float32)
     lib().orc_tgtp_rows(n, m, _p(idx, C.c_uint32), _p(adj, C.c_uint32), _p(t, C.c_int32),
                         epsilon, delta, gamma, src_begin, src_end, threads, _p(out, C.c_float))
+    return out
+
+
+def tgtp_sources(idx, adj, taus_arr, epsilon: float, gamma: int, sources, delta=None,
+                 threads: int = 0) -> np.ndarray:
+    """Rows (preds before symmetrization) of the listed sources only; other
+    entries are 0.  score(pos) = rows[pos] + rows[rev(pos)] for an edge whose
+    endpoints are both listed."""
+    idx, adj = _u32(idx), _u32(adj)
+    t = np.ascontiguousarray(taus_arr, dtype=np.int32)
+    src = _u32(sources)
+    n, m = idx.shape[0] - 1, adj.shape[0]
+    delta = 1.0 / n if delta is None else delta
+    out = np.zeros(m, dtype=np.float32)
+    rc = lib().orc_tgtp_sources(n, m, _p(idx, C.c_uint32), _p(adj, C.c_uint32),
+                                _p(t, C.c_int32), epsilon, delta, gamma, _p(src, C.c_uint32),
+                                src.shape[0], threads, _p(out, C.c_float))
+    if rc != 0:
+        raise ValueError("orc_tgtp_sources: gamma < 1 or a source id >= n")
     return out
 
 

@@ -125,6 +125,14 @@ int orc_tgtp_rows(uint64_t n, uint64_t m, const uint32_t* idx,
                   double delta, int gamma, uint32_t src_begin,
                   uint32_t src_end, int threads, float* preds_rows);
 
+/* Same for an arbitrary list of sources (count entries, each < n); the
+ * score of an edge whose endpoints are both listed is
+ * preds_rows[pos] + preds_rows[first reverse position] (aesc.h:532-550). */
+int orc_tgtp_sources(uint64_t n, uint64_t m, const uint32_t* idx,
+                     const uint32_t* adj, const int32_t* taus, double epsilon,
+                     double delta, int gamma, const uint32_t* sources,
+                     uint64_t count, int threads, float* preds_rows);
+
 /* graph.h:29-97 (undirected, skip = 0).  Allocates *idx [n+1] and *adj [m]
  * with malloc; caller frees with orc_free.  Returns 0 or -1 (message in
  * orc_last_error). */

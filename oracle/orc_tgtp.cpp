@@ -360,6 +360,26 @@ extern "C" int orc_tgtp_rows(uint64_t n, uint64_t m, const uint32_t* idx,
   return 0;
 }
 
+extern "C" int orc_tgtp_sources(uint64_t n, uint64_t m, const uint32_t* idx,
+                                const uint32_t* adj, const int32_t* taus, double epsilon,
+                                double delta, int gamma, const uint32_t* sources,
+                                uint64_t count, int threads, float* preds_rows) {
+  if (gamma < 1) return -1;
+  for (uint64_t i = 0; i < count; ++i)
+    if (sources[i] >= n) return -1;
+  const Csr g{n, m, idx, adj};
+  const Params P{epsilon, delta, std::log(m / delta), gamma, ORC_WALK_SABA};
+  const int nt = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel num_threads(nt)
+  {
+    SourceScratch S(n);
+#pragma omp for schedule(dynamic)
+    for (int64_t i = 0; i < static_cast<int64_t>(count); ++i)
+      run_source(g, taus, P, sources[i], preds_rows, S, nullptr, nullptr);
+  }
+  return 0;
+}
+
 // graph_t::load_txt, graph.h:29-97 (undirected, no skipped lines).
 extern "C" int orc_load_txt(const char* path, uint64_t* n_out, uint64_t* m_out,
                             uint32_t** idx_out, uint32_t** adj_out) {

@@ -11,7 +11,8 @@ import numpy as np
 import pytest
 
 from oracle.exact import exact_resistance
-from tgtp_amd import MODE_FAITHFUL, MODE_FAST_BF16, MODE_FAST_FP32
+import tau_rules as TR
+from tgtp_amd import MODE_FAITHFUL, MODE_FAST_BF16, MODE_FAST_FP32, TAU_REFERENCE
 from tgtp_amd import eigenprep as E
 from tgtp_amd import graph as G
 from tgtp_amd import workloads as W
@@ -59,17 +60,46 @@ def test_fast_within_eps_of_exact(gpu_ctx, cl4096, eps, mode, name):
 
 @pytest.mark.parametrize("mode,name", MODES)
 def test_mfma_taus_match_faithful(gpu_ctx, cl4096, mode, name):
-    """The projection changes only rounding: taus equal the faithful ones
-    except where the bound sits on an integer boundary (<= 1% FP32, <= 5%
-    BF16 of CSR positions)."""
+    """With the reference's refinement rule the projection changes only
+    rounding: taus equal the faithful ones except where the bound sits on an
+    integer boundary (<= 1% FP32, <= 5% BF16 of CSR positions)."""
     g, lam, X, _ = cl4096
     for eps in (0.05, 0.01):
         tf, _ = gpu_ctx.taus(g, lam, X, np.float32(eps / 2), MODE_FAITHFUL)
-        tq, _ = gpu_ctx.taus(g, lam, X, np.float32(eps / 2), mode)
+        tq, _ = gpu_ctx.taus(g, lam, X, np.float32(eps / 2), mode | TAU_REFERENCE)
         diff = np.mean(tf != tq)
         _say(f"  taus {name} eps={eps}: {diff * 100:.3f}% differ, max |d| {np.abs(tf - tq).max()}")
         assert diff <= (0.01 if mode == MODE_FAST_FP32 else 0.05)
         assert np.abs(tf - tq).max() <= 2
+
+
+@pytest.fixture(scope="module")
+def batch_bridges():
+    """Batched-workload graphs 29 and 156: edges whose top-omega tail alone
+    exceeds eps/2 (the reference's rule gives them tau = 1)."""
+    gs = W.batch_graphs(157)
+    return [(gs[i], *E.eigens_f32(*E.compute_eigens(gs[i], 128))) for i in (29, 156)]
+
+
+@pytest.mark.parametrize("mode,name", MODES)
+def test_mfma_taus_bounded_rule(gpu_ctx, cl4096, batch_bridges, mode, name):
+    """The default (bounded) rule: taus equal the numpy restatement of that
+    rule except at rounding boundaries (FP32 <= 1% of positions, BF16 <= 5%),
+    and every tau keeps the FP64 truncation bound within the projection's
+    rounding (delta and upsilon carry ~1e-6 relative error in FP32, ~1e-2 in
+    BF16)."""
+    slack = 1e-3 if mode == MODE_FAST_FP32 else 5e-2
+    cases = [(g, lam, X) for g, lam, X in batch_bridges] + [cl4096[:3]]
+    for g, lam, X in cases:
+        for eps in (0.05, 0.01):
+            tq, _ = gpu_ctx.taus(g, lam, X, np.float32(eps / 2), mode)
+            tn = TR.taus(g, lam, X, eps / 2, bounded=True)
+            diff = np.mean(tq != tn)
+            bound = TR.truncation_bound(g, lam, X, tq)
+            _say(f"  bounded taus {name} n={g.n} eps={eps}: {diff * 100:.3f}% differ from the "
+                 f"restatement, max bound {bound.max():.5f} (eps/2 {eps / 2})")
+            assert diff <= (0.01 if mode == MODE_FAST_FP32 else 0.05)
+            assert bound.max() <= eps / 2 * (1 + slack)
 
 
 def test_fast_symmetric_output_and_trace(gpu_ctx):

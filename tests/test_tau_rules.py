@@ -1,0 +1,94 @@
+"""Tau refinement rules (calculate_taus, src/aesc.h:129-216) on the CPU.
+
+* The numpy restatement (tests/tau_rules.py) with the reference's rule equals
+  the oracle's taus bit for bit (the oracle equals the reference build's).
+* The truncation bound it checks against is pinned by the full spectrum: on
+  small graphs the missed part of the series, summed over all eigenpairs,
+  never exceeds it, and the full series gives the exact resistance.
+* The reference's rule leaves that bound on edges of the batched workload's
+  graphs (the top-omega tail is dropped from the bound when it reaches
+  eps/2, aesc.h:130); the bounded rule of the FAST modes keeps it on every
+  edge.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import tau_rules as TR
+from tgtp_amd import eigenprep as E, exact_sc, graph as G, workloads as WL
+
+
+@pytest.fixture(scope="module")
+def batch_small():
+    # batch graphs 29, 156 (bridges into sparse parts: reference taus of 1
+    # where the top-omega tail alone is ~0.5) and 4 (dense, no such edge)
+    gs = WL.batch_graphs(157)
+    out = {}
+    for gi in (4, 29, 156):
+        g = gs[gi]
+        out[gi] = (g, *E.eigens_f32(*E.compute_eigens(g, 128)))
+    return out
+
+
+@pytest.mark.parametrize("gi", [4, 29, 156])
+@pytest.mark.parametrize("eps", [0.05, 0.01])
+def test_reference_rule_equals_oracle(batch_small, gi, eps):
+    g, lam, X = batch_small[gi]
+    t_np = TR.taus(g, lam, X, eps / 2, bounded=False)
+    t_or = oracle.taus(g.idx, g.adj, lam, X, np.float32(eps / 2))
+    assert np.array_equal(t_np, t_or)
+
+
+@pytest.mark.parametrize("seed", [3, 8])
+def test_truncation_bound_holds_on_full_spectrum(seed):
+    """Every eigenpair of a small graph: R = sum (y.v_i)^2 / (1 - lambda_i)
+    equals the exact resistance (pins the D_i / m normalisation), and the
+    missed part after tau steps is below truncation_bound for odd tau."""
+    g = G.chung_lu(180, 2.4, 6.0, seed=seed)
+    omega = 24
+    lam_all, X_all = E.compute_eigens(g, g.n)  # every pair, sorted by |lambda|
+    R = exact_sc.exact_scores(g, "cpu")
+    rows, und = TR._edges(g)
+    U, V = rows[und], g.adj[und].astype(np.int64)
+    D = (X_all[U, 1:] - X_all[V, 1:]) ** 2 / g.m
+    li = lam_all[1:]
+    assert np.allclose(D @ (1.0 / (1.0 - li)), R[und], rtol=1e-9, atol=1e-12)
+    lam, X = lam_all[:omega], X_all[:, :omega]
+    for tau in (1, 3, 7, 15):
+        missed = D @ (li ** (tau + 1) / (1.0 - li))
+        bound = TR.truncation_bound(g, lam, X, np.full(g.m, tau))
+        assert (missed <= bound * (1 + 1e-9) + 1e-15).all()
+
+
+@pytest.mark.parametrize("eps", [0.05, 0.01])
+def test_bounded_rule_keeps_the_bound_reference_rule_leaves_it(batch_small, eps):
+    for gi, (g, lam, X) in batch_small.items():
+        tb = TR.taus(g, lam, X, eps / 2, bounded=True)
+        tr = TR.taus(g, lam, X, eps / 2, bounded=False)
+        assert (TR.truncation_bound(g, lam, X, tb) <= eps / 2 * (1 + 1e-6)).all()
+        # the bounded rule only ever takes more steps
+        assert (tb >= tr).all()
+        viol = TR.truncation_bound(g, lam, X, tr) > eps / 2 * (1 + 1e-6)
+        if gi == 4:
+            assert not viol.any()
+        else:
+            # the reference rule leaves up to ~0.5 (graph 156) / ~0.1-0.3 (graph 29) unbounded
+            assert viol.sum() >= 100
+            assert TR.truncation_bound(g, lam, X, tr).max() > 0.1
+
+
+def test_reference_algorithm_error_on_bridge_edges(batch_small):
+    """The reference algorithm itself (oracle, bit-exact to the reference
+    build) misses eps on graph 156 by ~0.5 at both eps: its tau is 1 on the
+    bridge (5, 1139); the same pipeline with bounded taus is within 0.055."""
+    g, lam, X = batch_small[156]
+    R = exact_sc.exact_scores(g, "cpu")
+    rev = g.reverse_index()
+    for eps in (0.05, 0.01):
+        ref = oracle.tgtp(g.idx, g.adj, lam, X, eps, 10)
+        assert np.abs(ref - R).max() > 0.45
+        pos = int(np.abs(ref - R).argmax())
+        assert oracle.taus(g.idx, g.adj, lam, X, np.float32(eps / 2))[pos] == 1
+        tb = TR.taus(g, lam, X, eps / 2, bounded=True)
+        rows = oracle.tgtp_rows(g.idx, g.adj, tb, eps, 10, 0, g.n)
+        assert np.abs(rows + rows[rev] - R).max() < 0.055

@@ -23,6 +23,8 @@ TGTP_E_UNSUPPORTED = -5
 MODE_FAITHFUL = 0
 MODE_FAST_FP32 = 1
 MODE_FAST_BF16 = 2
+# FAST modes: the reference's tau refinement rule (aesc.h:130) instead of the bounded one
+TAU_REFERENCE = 0x100
 
 WALK_MT_REGULAR = 0
 WALK_MT_SIMD = 1

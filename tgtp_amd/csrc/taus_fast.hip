@@ -11,7 +11,10 @@
 // v_mfma_f32_16x16x32_bf16 (FAST_BF16: eigenvectors stored and D, W rounded
 // to BF16).  The stopping test (get_tau, aesc.h:129-135) runs in FP64 on the
 // tile's row after each 16-column tile; further column tiles are computed
-// only while some edge of the tile is still refining.
+// only while some edge of the tile is still refining.  With A.bounded a
+// refinement step whose delta alone reaches eps/2 is skipped (t advances,
+// tau is kept) instead of being bounded against eps/2 (aesc.h:130), so the
+// kept tau bounds both the top-omega tail and the rest of the series.
 //
 // Lane mapping (16x16 MFMA): lane l feeds row / column l % 16 and the k
 // range of group l / 16; k ranges of A and B are the same permutation of
@@ -141,9 +144,8 @@ __global__ void __launch_bounds__(kTB) taus_fast_kernel(TauFastArgs A) {
 #pragma unroll
         for (uint32_t s = 0; s < 4; ++s) {
           if (8 * s < kq) {
-            bf8 wb;
-#pragma unroll
-            for (uint32_t q = 0; q < 8; ++q) wb[q] = wcol[8 * s + q];
+            // one 16-byte load: omega_pad is a multiple of 32 and k0 of 8
+            const bf8 wb = *reinterpret_cast<const bf8*>(wcol + 8 * s);
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(dbf[s], wb, acc, 0, 0, 0);
           }
         }
@@ -162,6 +164,14 @@ __global__ void __launch_bounds__(kTB) taus_fast_kernel(TauFastArgs A) {
         }
         for (; c < 16 && !done; ++c) {
           const float del = ct[w][l][c] / G.m;
+          if (A.bounded && !(G.eps - del > 0.0)) {
+            // the top-omega tail after t steps alone reaches eps/2, so no
+            // tau >= t bounds the series with this delta; go on to the next
+            // step (aesc.h:130 would bound against eps/2 alone here)
+            t += 2;
+            if (t >= tau) done = true;
+            continue;
+          }
           const int tp = tau_bound64(du, dv, G.eps, G.sq1m_w, G.b_w, del, ups);
           if (t <= tp && tp < tau) {
             tau = tp;

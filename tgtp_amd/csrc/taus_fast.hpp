@@ -31,6 +31,7 @@ struct TauFastArgs {
   uint32_t n;                 // total vertices
   int32_t omega;
   uint32_t omega_pad;         // multiple of 32, <= 128
+  uint32_t bounded;           // 1: skip refinement steps whose top-omega tail reaches eps/2
   int32_t* taus;
 };
 

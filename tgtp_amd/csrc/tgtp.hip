@@ -910,15 +910,14 @@ static int validate_params(tgtp_ctx* ctx, const tgtp_params* p) {
   if (!(p->epsilon > 0 && p->epsilon < 1) || p->gamma < 1 || !(p->delta > 0))
     return set_error(ctx, TGTP_E_ARG,
                      "require 0 < epsilon < 1, integer omega >= 2, integer gamma >= 1, delta > 0");
-  if (p->mode != TGTP_MODE_FAITHFUL && p->mode != TGTP_MODE_FAST_FP32 &&
-      p->mode != TGTP_MODE_FAST_BF16)
-    return set_error(ctx, TGTP_E_ARG, "unknown tgtp mode");
+  if (!tgtp_mode_valid(p->mode)) return set_error(ctx, TGTP_E_ARG, "unknown tgtp mode");
   return TGTP_OK;
 }
 
 static int run_score(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e, const tgtp_params* p,
                      float* out, tgtp_stats* stats, const TraceOut* tr) {
-  if (p->mode == TGTP_MODE_FAITHFUL) return run_score_faithful(ctx, dg, e, p, out, stats, tr);
+  if (tgtp_mode_base(p->mode) == TGTP_MODE_FAITHFUL)
+    return run_score_faithful(ctx, dg, e, p, out, stats, tr);
   return tgtp_run_score_fast(ctx, dg, e, p, out, stats, tr);
 }
 
@@ -980,8 +979,7 @@ extern "C" int tgtp_taus(tgtp_ctx* ctx, const tgtp_graph* g, const tgtp_eigens* 
                          float eps_half, int32_t mode, int32_t* taus_out, tgtp_stats* stats) {
   if (!ctx) return TGTP_E_ARG;
   if (!taus_out) return set_error(ctx, TGTP_E_ARG, "null output");
-  if (mode != TGTP_MODE_FAITHFUL && mode != TGTP_MODE_FAST_FP32 && mode != TGTP_MODE_FAST_BF16)
-    return set_error(ctx, TGTP_E_ARG, "unknown tgtp mode");
+  if (!tgtp_mode_valid(mode)) return set_error(ctx, TGTP_E_ARG, "unknown tgtp mode");
   if (!(eps_half > 0)) return set_error(ctx, TGTP_E_ARG, "epsilon must be positive");
   const double t0 = now_ms();
   tgtp_dgraph* dg = nullptr;
@@ -998,7 +996,7 @@ extern "C" int tgtp_taus(tgtp_ctx* ctx, const tgtp_graph* g, const tgtp_eigens* 
   hipStream_t s = ctx->stream;
   if (hipEventRecord(ctx->ev[4], s) != hipSuccess) return cleanup(TGTP_E_HIP);
   uint64_t fixups = 0;
-  rc = mode == TGTP_MODE_FAITHFUL ? tgtp_run_taus_faithful(ctx, dg, e, eps_half, d_taus, &fixups)
+  rc = tgtp_mode_base(mode) == TGTP_MODE_FAITHFUL ? tgtp_run_taus_faithful(ctx, dg, e, eps_half, d_taus, &fixups)
                                  : tgtp_run_taus_fast(ctx, dg, e, eps_half, mode, d_taus);
   if (rc) return cleanup(rc);
   if (hipEventRecord(ctx->ev[5], s) != hipSuccess ||

@@ -253,7 +253,7 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
   if ((rc = dev_upload(ctx, &P->d_wt, wt, "tau column tables")) ||
       (rc = dev_upload(ctx, &P->d_tgraphs, tg, "tau graph params")))
     return fail(rc);
-  if (mode == TGTP_MODE_FAST_BF16) {
+  if (tgtp_mode_base(mode) == TGTP_MODE_FAST_BF16) {
     if ((rc = dev_alloc(ctx, &P->d_wtb, wt.size(), "bf16 tables")) ||
         (rc = launch_to_bf16(ctx, P->d_wt, wt.size(), P->d_wtb)) ||
         (rc = dev_alloc(ctx, &P->d_xb, dg->n * omega, "bf16 eigenvectors")) ||
@@ -287,7 +287,8 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
   // taus once, to bound the rounds each block can take (a source stops by
   // the round equal to the largest tau of its row, when no walk is left)
   if ((rc = launch_fill_i32(ctx, P->d_taus, P->m, 0)) ||
-      (rc = launch_taus_fast(ctx, plan_tau_args(P, dg), mode == TGTP_MODE_FAST_BF16)))
+      (rc = launch_taus_fast(ctx, plan_tau_args(P, dg),
+                             tgtp_mode_base(mode) == TGTP_MODE_FAST_BF16)))
     return fail(rc);
   std::vector<int32_t> taus(dg->m);
   if (dg->m)
@@ -518,6 +519,7 @@ TauFastArgs plan_tau_args(const FastPlan* P, tgtp_dgraph* dg) {
   T.n = static_cast<uint32_t>(dg->n);
   T.omega = P->omega;
   T.omega_pad = P->omega_pad;
+  T.bounded = (P->mode & TGTP_TAU_REFERENCE) ? 0u : 1u;
   T.taus = P->d_taus;
   return T;
 }
@@ -529,7 +531,8 @@ int enqueue_run(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_
   int rc;
   if ((rc = launch_fill_words(ctx, P->d_words, 16)) ||
       (rc = launch_fill_i32(ctx, P->d_taus, P->m, 0)) ||
-      (rc = launch_taus_fast(ctx, plan_tau_args(P, dg), P->mode == TGTP_MODE_FAST_BF16)))
+      (rc = launch_taus_fast(ctx, plan_tau_args(P, dg),
+                             tgtp_mode_base(P->mode) == TGTP_MODE_FAST_BF16)))
     return rc;
   if (record_events) TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
   BlockArgs A = override_args ? *override_args : plan_args(P, dg);
@@ -768,7 +771,8 @@ int tgtp_run_taus_fast(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e, flo
   if (rc) return rc;
   // float(p.epsilon / 2) == eps_half exactly (doubling and halving are exact)
   if ((rc = launch_fill_i32(ctx, P->d_taus, P->m, 0)) ||
-      (rc = launch_taus_fast(ctx, plan_tau_args(P, dg), mode == TGTP_MODE_FAST_BF16)))
+      (rc = launch_taus_fast(ctx, plan_tau_args(P, dg),
+                             tgtp_mode_base(mode) == TGTP_MODE_FAST_BF16)))
     return rc;
   TGTP_HIP_CHECK(ctx, hipMemcpyAsync(d_taus, P->d_taus, dg->m * 4, hipMemcpyDeviceToDevice,
                                      ctx->stream));
@@ -815,6 +819,7 @@ extern "C" int tgtp_batch_create(tgtp_ctx* ctx, int32_t count, const tgtp_graph*
       return set_error(ctx, TGTP_E_ARG,
                        "require 0 < epsilon < 1, integer gamma >= 1, delta > 0 for every graph");
     if (p.mode != mode) return set_error(ctx, TGTP_E_ARG, "all graphs of a batch need one mode");
+    if (!tgtp_mode_valid(p.mode)) return set_error(ctx, TGTP_E_ARG, "unknown tgtp mode");
     if (!graphs[k].idx || graphs[k].n == 0 || !eigens[k].values || !eigens[k].vectors ||
         eigens[k].omega < 2 || static_cast<uint64_t>(eigens[k].omega) > graphs[k].n)
       return set_error(ctx, TGTP_E_ARG, "graph or eigenpairs missing / omega out of range");
@@ -843,7 +848,7 @@ extern "C" int tgtp_batch_create(tgtp_ctx* ctx, int32_t count, const tgtp_graph*
     return rc;
   };
   int rc;
-  if (mode == TGTP_MODE_FAITHFUL) {
+  if (tgtp_mode_base(mode) == TGTP_MODE_FAITHFUL) {
     for (int32_t k = 0; k < count; ++k) {
       tgtp_dgraph* d = nullptr;
       if ((rc = tgtp_graph_upload(ctx, &graphs[k], &d))) return fail(rc);
@@ -914,7 +919,7 @@ extern "C" int tgtp_batch_run(tgtp_ctx* ctx, tgtp_batch* b, float* const* scores
   const double t0 = tgtp_now_ms();
   TGTP_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   int rc;
-  if (b->mode == TGTP_MODE_FAITHFUL) {
+  if (tgtp_mode_base(b->mode) == TGTP_MODE_FAITHFUL) {
     double kms = 0;
     for (int32_t k = 0; k < b->count; ++k) {
       tgtp_stats st{};

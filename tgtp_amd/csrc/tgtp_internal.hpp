@@ -28,6 +28,14 @@ struct TraceOut {
   uint32_t* q_size;
 };
 
+// tgtp_params.mode = TGTP_MODE_* | optional TGTP_TAU_REFERENCE
+inline int32_t tgtp_mode_base(int32_t mode) { return mode & 0xff; }
+inline bool tgtp_mode_valid(int32_t mode) {
+  const int32_t b = tgtp_mode_base(mode);
+  return (mode & ~(0xff | TGTP_TAU_REFERENCE)) == 0 &&
+         (b == TGTP_MODE_FAITHFUL || b == TGTP_MODE_FAST_FP32 || b == TGTP_MODE_FAST_BF16);
+}
+
 int tgtp_ensure_prep(tgtp_ctx* ctx, tgtp_dgraph* dg);
 int tgtp_ensure_eigens(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e);
 int tgtp_check_degrees(tgtp_ctx* ctx, tgtp_dgraph* dg);
