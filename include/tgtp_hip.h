@@ -203,6 +203,24 @@ int tgtp_score_batch(tgtp_ctx* ctx, int32_t count, const tgtp_graph* graphs,
                      const tgtp_eigens* eigens, const tgtp_params* params,
                      float* const* scores_out, tgtp_stats* stats);
 
+/* ---- source subsets ----------------------------------------------------------
+ * No counterpart in the reference, which scores every source; it serves
+ * sampled measurements of graphs whose full run takes minutes (the
+ * reference's per-source work is independent, aesc.h:456-527, so a uniform
+ * sample of sources extrapolates).  tgtp_score_dev restricted to the sources
+ * of n_ranges half-open ranges ranges[2k] .. ranges[2k+1] (non-empty,
+ * ascending, disjoint, within [0, n)): taus for every edge, the per-source
+ * pipeline for the listed sources only.  rows_out [m] (nullable) receives
+ * the row value of every CSR position whose source is listed (preds before
+ * symmetrization, aesc.h:532: FAITHFUL the float row, bit for bit the full
+ * run's; FAST the FP64 row sum rounded to float); scores_out [m] (nullable)
+ * the score of every position whose source and target are both listed,
+ * equal to what the full run returns there.  All other entries are left
+ * unchanged.  stats->work_items = sources run. */
+int tgtp_score_sources(tgtp_ctx* ctx, const tgtp_dgraph* dg, const tgtp_eigens* e,
+                       const tgtp_params* p, uint32_t n_ranges, const uint32_t* ranges,
+                       float* rows_out, float* scores_out, tgtp_stats* stats);
+
 /* Diagnostic form of tgtp_score_dev that also returns the per-stage values
  * of the pipeline (each output pointer may be NULL): taus [m] (calculate_taus,
  * aesc.h:164-216), nr [m] walk count per position (0 where tau <= ell,

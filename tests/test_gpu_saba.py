@@ -130,20 +130,35 @@ def test_invalid_arguments_rejected(gpu_ctx):
     dg.free()
 
 
-def test_full_scale_chunglu_1m(gpu_ctx):
-    """Walk config at full size (n=2^20, ~20M CSR entries, 10 hops, 16384
-    walks/vertex): size-independent invariant on the whole run, bit-exact
-    counters on source slices the oracle finishes in seconds."""
-    g = W.chung_lu_1m()
-    _say(f"cl1m n={g.n} m={g.m}")
+@pytest.fixture(scope="module")
+def cl1m():
+    return W.chung_lu_1m()
+
+
+@pytest.mark.parametrize("depth", [10, 15])
+def test_full_scale_chunglu_1m(gpu_ctx, cl1m, depth):
+    """Walk config at full size (n=2^20, ~20M CSR entries, 10 and 15 hops,
+    16384 walks/vertex): size-independent invariants on the whole run (visit
+    count n*W*L, checksum of the counters, the counters of four disjoint
+    source quarters summing to the whole run's), bit-exact counters on
+    source slices the oracle finishes in seconds."""
+    g = cl1m
+    _say(f"cl1m n={g.n} m={g.m} depth={depth}")
     dg = gpu_ctx.upload(g)
     try:
-        cnt, vc, cs, st = gpu_ctx.saba_walks(dg, 10, 16384)
-        assert vc == g.n * 16384 * 10 and _sums(cnt) == (vc, cs)
+        cnt, vc, cs, st = gpu_ctx.saba_walks(dg, depth, 16384)
+        assert vc == g.n * 16384 * depth and _sums(cnt) == (vc, cs)
+        acc = np.zeros(g.n, dtype=np.uint64)
+        q = g.n // 4
+        for k in range(4):
+            part, _, _, _ = gpu_ctx.saba_walks(dg, depth, 16384, k * q, (k + 1) * q)
+            acc += part
+        assert np.array_equal(acc, cnt.astype(np.uint64)), "quarters do not sum to the run"
         for lo, hi in [(0, 384), (g.n // 2, g.n // 2 + 512), (g.n - 640, g.n)]:
-            part, pvc, pcs, _ = gpu_ctx.saba_walks(dg, 10, 16384, lo, hi)
-            ref = oracle.saba_walks(g.idx, g.adj, 10, 16384, lo, hi)
+            part, pvc, pcs, _ = gpu_ctx.saba_walks(dg, depth, 16384, lo, hi)
+            ref = oracle.saba_walks(g.idx, g.adj, depth, 16384, lo, hi)
             assert np.array_equal(part, ref), f"slice [{lo},{hi})"
+            assert (pvc, pcs) == _sums(ref)
     finally:
         dg.free()
 

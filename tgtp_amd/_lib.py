@@ -73,7 +73,7 @@ class StatsStruct(C.Structure):
 EXPORTED = [
     "tgtp_ctx_create", "tgtp_ctx_destroy", "tgtp_ctx_last_error", "tgtp_strerror",
     "tgtp_graph_upload", "tgtp_graph_free", "tgtp_saba_walks", "tgtp_mt_walks", "tgtp_taus",
-    "tgtp_score", "tgtp_score_dev", "tgtp_score_trace",
+    "tgtp_score", "tgtp_score_dev", "tgtp_score_trace", "tgtp_score_sources",
     "tgtp_batch_create", "tgtp_batch_run", "tgtp_batch_destroy", "tgtp_batch_config",
     "tgtp_score_batch",
     "tgtp_load_txt", "tgtp_csr_free", "tgtp_save_csr_bin", "tgtp_load_csr_bin",
@@ -130,6 +130,8 @@ def lib() -> C.CDLL:
                                    P(StatsStruct), P(C.c_int32), P(C.c_uint64), P(C.c_float),
                                    P(C.c_float), P(C.c_int32), P(C.c_float), P(C.c_float),
                                    P(C.c_float), P(C.c_uint32)]
+    L.tgtp_score_sources.argtypes = [vp, vp, P(EigensStruct), P(ParamsStruct), C.c_uint32,
+                                     P(C.c_uint32), P(C.c_float), P(C.c_float), P(StatsStruct)]
     L.tgtp_batch_create.argtypes = [vp, C.c_int32, P(GraphStruct), P(EigensStruct),
                                     P(ParamsStruct), C.c_int32, P(vp)]
     L.tgtp_batch_run.argtypes = [vp, vp, P(P(C.c_float)), P(StatsStruct)]

@@ -94,6 +94,7 @@ enum BufSlot {
   kBufTrace = 11,
   kBufWalkRec = 12,
   kBufWalkFill = 13,
+  kBufSubset = 14,  // source list of tgtp_score_sources
 };
 
 // Returns a device buffer of at least `bytes` in slot `slot` (grow-only).

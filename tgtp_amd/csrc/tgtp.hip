@@ -213,6 +213,8 @@ struct SrcArgs {
   uint32_t* next_src;
   uint32_t* n_ambiguous;
   uint32_t* status;   // bit 0: a walk count exceeded kMaxWalks
+  const uint32_t* src_list;  // sources to run (null: all n, in id order)
+  uint32_t n_src;            // entries of src_list
   uint32_t n;
   int32_t gamma;
   double eps;
@@ -556,12 +558,13 @@ __global__ void __launch_bounds__(64) source_faithful_kernel(SrcArgs A) {
   uint32_t* qb = qa + n;
   for (uint32_t i = lane_id(); i < n; i += 64) seen[i] = 0;
   __syncthreads();
+  const uint32_t count = A.src_list ? A.n_src : n;
   for (;;) {
-    uint32_t src = 0;
-    if (lane_id() == 0) src = atomicAdd(A.next_src, 1u);
-    src = __shfl(src, 0);
-    if (src >= n) break;
-    faithful_source<kLds>(A, src, pvec, seen, qa, qb);
+    uint32_t k = 0;
+    if (lane_id() == 0) k = atomicAdd(A.next_src, 1u);
+    k = __shfl(k, 0);
+    if (k >= count) break;
+    faithful_source<kLds>(A, A.src_list ? A.src_list[k] : k, pvec, seen, qa, qb);
   }
 }
 
@@ -791,9 +794,16 @@ int tgtp_run_taus_faithful(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e,
   return TGTP_OK;
 }
 
+// sl (optional): run only the listed sources and also return every row.
+struct SourceList {
+  const uint32_t* d_list;
+  uint32_t count;
+  float* rows_host;  // [m] preds before symmetrization
+};
+
 static int run_score_faithful(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e,
                               const tgtp_params* prm, float* scores_out, tgtp_stats* stats,
-                              const TraceOut* tr) {
+                              const TraceOut* tr, const SourceList* sl = nullptr) {
   const double t_host0 = now_ms();
   hipStream_t s = ctx->stream;
   const uint64_t n = dg->n, m = dg->m;
@@ -815,8 +825,9 @@ static int run_score_faithful(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens*
   uint32_t per_cu = use_lds ? static_cast<uint32_t>(std::min<size_t>(8, (160 * 1024) / lds_bytes))
                             : 4u;
   per_cu = std::max<uint32_t>(per_cu, 1);
+  const uint64_t n_run = sl ? std::max<uint32_t>(sl->count, 1u) : n;
   const uint32_t slots =
-      static_cast<uint32_t>(std::min<uint64_t>(n, static_cast<uint64_t>(ctx->num_cus) * per_cu));
+      static_cast<uint32_t>(std::min<uint64_t>(n_run, static_cast<uint64_t>(ctx->num_cus) * per_cu));
   auto* d_q = static_cast<uint32_t*>(ctx_buffer(ctx, kBufQueue, static_cast<uint64_t>(slots) * 2 * n * 4));
   auto* d_words = static_cast<uint32_t*>(ctx_buffer(ctx, kBufSums, 64));
   float* d_pv = nullptr;
@@ -839,6 +850,8 @@ static int run_score_faithful(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens*
   A.next_src = d_words;
   A.n_ambiguous = d_words + 1;
   A.status = d_words + 2;
+  A.src_list = sl ? sl->d_list : nullptr;
+  A.n_src = sl ? sl->count : 0u;
   A.n = static_cast<uint32_t>(n);
   A.gamma = prm->gamma;
   A.eps = prm->epsilon;
@@ -866,6 +879,7 @@ static int run_score_faithful(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens*
   TGTP_HIP_CHECK(ctx, hipGetLastError());
   TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[7], s));
   TGTP_HIP_CHECK(ctx, hipMemcpyAsync(scores_out, d_out, m * 4, hipMemcpyDeviceToHost, s));
+  if (sl) TGTP_HIP_CHECK(ctx, hipMemcpyAsync(sl->rows_host, d_preds, m * 4, hipMemcpyDeviceToHost, s));
   uint32_t words[3] = {0, 0, 0};
   TGTP_HIP_CHECK(ctx, hipMemcpyAsync(words, d_words, 12, hipMemcpyDeviceToHost, s));
   if (tr) {
@@ -942,6 +956,66 @@ extern "C" int tgtp_score_dev(tgtp_ctx* ctx, const tgtp_dgraph* cdg, const tgtp_
   if ((rc = tgtp_check_degrees(ctx, dg))) return rc;
   if ((rc = tgtp_ensure_eigens(ctx, dg, e))) return rc;
   return run_score(ctx, dg, e, p, scores_out, stats, nullptr);
+}
+
+extern "C" int tgtp_score_sources(tgtp_ctx* ctx, const tgtp_dgraph* cdg, const tgtp_eigens* e,
+                                  const tgtp_params* p, uint32_t n_ranges,
+                                  const uint32_t* ranges, float* rows_out, float* scores_out,
+                                  tgtp_stats* stats) {
+  if (!ctx) return TGTP_E_ARG;
+  if (!cdg || !ranges || n_ranges == 0)
+    return set_error(ctx, TGTP_E_ARG, "null graph or empty source ranges");
+  auto* dg = const_cast<tgtp_dgraph*>(cdg);
+  int rc = validate_params(ctx, p);
+  if (rc) return rc;
+  std::vector<std::pair<uint32_t, uint32_t>> rg(n_ranges);
+  uint64_t count = 0;
+  for (uint32_t k = 0; k < n_ranges; ++k) {
+    rg[k] = {ranges[2 * k], ranges[2 * k + 1]};
+    if (rg[k].first >= rg[k].second || rg[k].second > dg->n ||
+        (k && rg[k].first < rg[k - 1].second))
+      return set_error(ctx, TGTP_E_ARG,
+                       "source ranges must be non-empty, ascending, disjoint and within [0, n)");
+    count += rg[k].second - rg[k].first;
+  }
+  TGTP_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  if ((rc = tgtp_ensure_prep(ctx, dg))) return rc;
+  if ((rc = tgtp_check_degrees(ctx, dg))) return rc;
+  if ((rc = tgtp_ensure_eigens(ctx, dg, e))) return rc;
+  const uint64_t m = dg->m;
+  std::vector<float> all_scores(m), all_rows(m);
+  if (tgtp_mode_base(p->mode) == TGTP_MODE_FAITHFUL) {
+    std::vector<uint32_t> list;
+    list.reserve(count);
+    for (const auto& r : rg)
+      for (uint32_t v = r.first; v < r.second; ++v) list.push_back(v);
+    auto* d_list = static_cast<uint32_t*>(ctx_buffer(ctx, kBufSubset, count * 4));
+    if (!d_list) return set_error(ctx, TGTP_E_NOMEM, "source list");
+    TGTP_HIP_CHECK(ctx, hipMemcpyAsync(d_list, list.data(), count * 4, hipMemcpyHostToDevice,
+                                       ctx->stream));
+    const SourceList sl{d_list, static_cast<uint32_t>(count), all_rows.data()};
+    rc = run_score_faithful(ctx, dg, e, p, all_scores.data(), stats, nullptr, &sl);
+  } else {
+    rc = tgtp_run_score_fast_sources(ctx, dg, e, p, rg, all_scores.data(), all_rows.data(), stats);
+  }
+  if (rc) return rc;
+  // keep rows of listed sources, scores of positions whose ends are both listed
+  const auto* P = static_cast<TgtpPrep*>(dg->prep);
+  auto listed = [&](uint32_t v) {
+    auto it = std::upper_bound(rg.begin(), rg.end(), v,
+                               [](uint32_t x, const std::pair<uint32_t, uint32_t>& r) {
+                                 return x < r.first;
+                               });
+    return it != rg.begin() && v < std::prev(it)->second;
+  };
+  for (const auto& r : rg)
+    for (uint32_t u = r.first; u < r.second; ++u)
+      for (uint32_t pos = P->h_idx[u]; pos < P->h_idx[u + 1]; ++pos) {
+        if (rows_out) rows_out[pos] = all_rows[pos];
+        if (scores_out && listed(P->h_adj[pos])) scores_out[pos] = all_scores[pos];
+      }
+  if (stats) stats->work_items = count;
+  return TGTP_OK;
 }
 
 extern "C" int tgtp_score(tgtp_ctx* ctx, const tgtp_graph* g, const tgtp_eigens* e,

@@ -159,9 +159,12 @@ int dev_upload(tgtp_ctx* ctx, T** p, const std::vector<T>& host, const char* wha
 TauFastArgs plan_tau_args(const FastPlan* P, tgtp_dgraph* dg);
 
 // Builds the device tables and scratch of a FAST run over `dg` (a single
-// graph or a union) whose eigenvectors are already in dg->X.
+// graph or a union) whose eigenvectors are already in dg->X.  With `ranges`
+// (a single graph only) the source blocks cover just those half-open source
+// ranges; taus still cover every edge.
 int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& specs, int32_t omega,
-               int32_t mode, FastPlan** out) {
+               int32_t mode, FastPlan** out,
+               const std::vector<std::pair<uint32_t, uint32_t>>* ranges = nullptr) {
   auto* prep = static_cast<TgtpPrep*>(dg->prep);
   auto* P = new (std::nothrow) FastPlan();
   if (!P) return set_error(ctx, TGTP_E_NOMEM, "host allocation");
@@ -307,24 +310,29 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
   });
   std::vector<BlockInfo> blocks;
   std::vector<uint32_t> bround;
+  uint64_t n_sources = 0;
   for (uint32_t k : order) {
     const GraphSpec& S = specs[k];
-    for (uint32_t s0 = 0; s0 < S.n; s0 += kBlkCols) {
-      BlockInfo B{};
-      B.voff = S.voff;
-      B.n = S.n;
-      B.src0 = s0;
-      B.cnt = std::min<uint32_t>(kBlkCols, S.n - s0);
-      B.gi = k;
-      int32_t tmax = 1;
-      for (uint32_t v = S.voff + s0; v < S.voff + s0 + B.cnt; ++v)
-        for (uint64_t p = idx[v]; p < idx[v + 1]; ++p) tmax = std::max(tmax, taus[p]);
-      blocks.push_back(B);
-      bround.push_back(static_cast<uint32_t>(tmax));
+    const std::vector<std::pair<uint32_t, uint32_t>> all{{0u, S.n}};
+    for (const auto& r : ranges ? *ranges : all) {
+      for (uint32_t s0 = r.first; s0 < r.second; s0 += kBlkCols) {
+        BlockInfo B{};
+        B.voff = S.voff;
+        B.n = S.n;
+        B.src0 = s0;
+        B.cnt = std::min<uint32_t>(kBlkCols, r.second - s0);
+        B.gi = k;
+        int32_t tmax = 1;
+        for (uint32_t v = S.voff + s0; v < S.voff + s0 + B.cnt; ++v)
+          for (uint64_t p = idx[v]; p < idx[v + 1]; ++p) tmax = std::max(tmax, taus[p]);
+        blocks.push_back(B);
+        bround.push_back(static_cast<uint32_t>(tmax));
+        n_sources += B.cnt;
+      }
     }
   }
   P->n_blocks = static_cast<uint32_t>(blocks.size());
-  P->n_sources = dg->n;
+  P->n_sources = n_sources;
   // tile order: per-XCD runs of whole blocks, or (a block's state larger
   // than 16 MiB, eight of which would not fit the Infinity Cache) groups of
   // 8 tiles of one block across the XCDs
@@ -777,6 +785,54 @@ int tgtp_run_taus_fast(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e, flo
   TGTP_HIP_CHECK(ctx, hipMemcpyAsync(d_taus, P->d_taus, dg->m * 4, hipMemcpyDeviceToDevice,
                                      ctx->stream));
   return TGTP_OK;
+}
+
+// Source subset: a plan whose blocks cover only `ranges` (not cached on the
+// dgraph), one run, every score and row copied back (the caller keeps the
+// positions it can use).
+int tgtp_run_score_fast_sources(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e,
+                                const tgtp_params* p,
+                                const std::vector<std::pair<uint32_t, uint32_t>>& ranges,
+                                float* scores_all, float* rows_all, tgtp_stats* stats) {
+  const double t0 = tgtp_now_ms();
+  if (e->omega > 128 || e->omega < 3)
+    return set_error(ctx, TGTP_E_UNSUPPORTED, "fast modes need 3 <= omega <= 128");
+  auto* prep = static_cast<TgtpPrep*>(dg->prep);
+  std::vector<GraphSpec> specs(1);
+  specs[0] = GraphSpec{0u, static_cast<uint32_t>(dg->n), 0ull, dg->m, e->values, p->epsilon,
+                       p->delta, p->gamma};
+  FastPlan* P = nullptr;
+  int rc = build_plan(ctx, dg, specs, e->omega, p->mode, &P, &ranges);
+  if (rc) return rc;
+  auto done = [&](int r) {
+    free_plan(P);
+    return r;
+  };
+  if ((rc = enable_dual(ctx, P))) return done(rc);
+  hipStream_t s = ctx->stream;
+  if (hipEventRecord(ctx->ev[4], s) != hipSuccess) return done(set_error(ctx, TGTP_E_HIP, "event"));
+  if ((rc = enqueue_run(ctx, P, dg, prep->d_src_of, true, nullptr))) return done(rc);
+  std::vector<double> pd(dg->m);
+  if (hipEventRecord(ctx->ev[7], s) != hipSuccess ||
+      hipMemcpyAsync(scores_all, P->d_out, dg->m * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(pd.data(), P->d_preds, dg->m * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return done(set_error(ctx, TGTP_E_HIP, "subset run"));
+  for (uint64_t i = 0; i < dg->m; ++i) rows_all[i] = static_cast<float>(pd[i]);
+  if ((rc = check_status(ctx, P))) return done(rc);
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    float a = 0, b = 0, c = 0;
+    (void)hipEventElapsedTime(&a, ctx->ev[4], ctx->ev[7]);
+    (void)hipEventElapsedTime(&b, ctx->ev[4], ctx->ev[5]);
+    (void)hipEventElapsedTime(&c, ctx->ev[5], ctx->ev[6]);
+    stats->kernel_ms = a;
+    stats->taus_ms = b;
+    stats->diffuse_ms = c;
+    stats->work_items = P->n_sources;
+    stats->e2e_ms = tgtp_now_ms() - t0;
+  }
+  return done(TGTP_OK);
 }
 
 // ---- batched C ABI ---------------------------------------------------------------

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cstdint>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
@@ -54,5 +55,9 @@ double tgtp_now_ms();
 int tgtp_run_score_fast(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e,
                         const tgtp_params* p, float* scores_out, tgtp_stats* stats,
                         const TraceOut* tr);
+int tgtp_run_score_fast_sources(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e,
+                                const tgtp_params* p,
+                                const std::vector<std::pair<uint32_t, uint32_t>>& ranges,
+                                float* scores_all, float* rows_all, tgtp_stats* stats);
 int tgtp_run_taus_fast(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens* e, float eps_half,
                        int32_t mode, int32_t* d_taus);

@@ -73,11 +73,31 @@ def _torch_csr(P, device):
                                    size=P.shape, dtype=torch.float64, device=device)
 
 
+def _tmm(A, B, chunk: int = 4096):
+    """A^T B for tall A (.., n, p), B (.., n, q) with n >> p, q: the long
+    dimension is cut into chunks whose partial products run as one batched
+    GEMM and are then summed (split-K).  One GEMM over all n rows keeps few
+    workgroups busy on such shapes: at n = 262,144, p = 1536, q = 64 it took
+    28 ms against 1 ms split."""
+    n = A.shape[-2]
+    S = n // chunk
+    if S < 2:
+        return A.mT @ B
+    h = S * chunk
+    lead = A.shape[:-2]
+    As = A[..., :h, :].reshape(*lead, S, chunk, A.shape[-1])
+    Bs = B[..., :h, :].reshape(*lead, S, chunk, B.shape[-1])
+    out = (As.mT @ Bs).sum(dim=-3)
+    if h < n:
+        out = out + A[..., h:, :].mT @ B[..., h:, :]
+    return out
+
+
 def _orth_against(Z, Q):
     # classical Gram-Schmidt, twice (CGS2): one pass loses orthogonality when
     # Z is nearly inside span(Q)
     for _ in range(2):
-        Z = Z - Q @ (Q.T @ Z)
+        Z = Z - Q @ _tmm(Q, Z)
     return Z
 
 
@@ -129,7 +149,7 @@ def block_krylov_eigs(P, omega: int, tol: float = 1e-9, block: int = 64, basis: 
                 nxt, _ = torch.linalg.qr(_orth_against(nxt, Q[:, :c]))
             if c + k > basis:
                 break
-        H = Q[:, :c].T @ W[:, :c]
+        H = _tmm(Q[:, :c], W[:, :c])
         H = 0.5 * (H + H.T)
         th, S = torch.linalg.eigh(H)
         order = torch.argsort(-th.abs(), stable=True)
@@ -154,7 +174,7 @@ def _cholqr2(Z):
 
     ok = torch.ones(Z.shape[0], dtype=torch.bool, device=Z.device)
     for _ in range(2):
-        G = Z.mT @ Z
+        G = _tmm(Z, Z)
         L, info = torch.linalg.cholesky_ex(0.5 * (G + G.mT))
         bad = info != 0
         ok &= ~bad
@@ -207,14 +227,14 @@ def batched_block_krylov_eigs(P, k: int, n_pad: int, n_real, omega: int, tol: fl
     def orth(Z, c):
         for _ in range(2):
             if c:
-                Z = Z - Q[:, :, :c] @ (Q[:, :, :c].mT @ Z)
+                Z = Z - Q[:, :, :c] @ _tmm(Q[:, :, :c], Z)
         Z, ok = _cholqr2(Z)
         if not bool(ok.all()):
             # an invariant subspace was reached: random directions instead
             Z = torch.where(ok[:, None, None], Z, rand(Z.shape[-1]))
             for _ in range(2):
                 if c:
-                    Z = Z - Q[:, :, :c] @ (Q[:, :, :c].mT @ Z)
+                    Z = Z - Q[:, :, :c] @ _tmm(Q[:, :, :c], Z)
             Z, _ = _cholqr2(Z)
         return Z
 
@@ -226,7 +246,7 @@ def batched_block_krylov_eigs(P, k: int, n_pad: int, n_real, omega: int, tol: fl
             W[:, :, c:c + block] = (P @ nxt.reshape(k * n_pad, block)).reshape(k, n_pad, block)
             c += block
             nxt = orth(W[:, :, c - block:c].clone(), c)
-        H = Q[:, :, :c].mT @ W[:, :, :c]
+        H = _tmm(Q[:, :, :c], W[:, :, :c])
         H = 0.5 * (H + H.mT)
         th, S = torch.linalg.eigh(H)
         order = torch.argsort(-th.abs(), dim=1, stable=True)

@@ -220,6 +220,26 @@ class Context:
         del keep
         return outs, st.as_dict()
 
+    def score_sources(self, dg: DeviceGraph, lam, X, epsilon: float, ranges, gamma: int = 10,
+                      delta: float | None = None, mode: int = L.MODE_FAST_FP32):
+        """tgtp_score_sources over half-open source ranges [(begin, end), ...]:
+        (rows, scores, stats); rows are NaN where the source is not listed,
+        scores NaN where the source or the target is not listed."""
+        es, keep = self._eigens(lam, X)
+        n, m = dg.n, dg.m
+        ps = L.ParamsStruct(epsilon, (1.0 / n) if delta is None else delta, gamma, mode)
+        rg = np.ascontiguousarray(np.asarray(ranges, dtype=np.uint32).reshape(-1))
+        rows = np.full(m, np.nan, dtype=np.float32)
+        scores = np.full(m, np.nan, dtype=np.float32)
+        st = L.StatsStruct()
+        rc = L.lib().tgtp_score_sources(self.ptr, dg.ptr, C.byref(es), C.byref(ps),
+                                        rg.shape[0] // 2, L.ptr(rg, C.c_uint32),
+                                        L.ptr(rows, C.c_float), L.ptr(scores, C.c_float),
+                                        C.byref(st))
+        _check(self.ptr, rc, "tgtp_score_sources")
+        del keep
+        return rows, scores, st.as_dict()
+
     def score_trace(self, dg: DeviceGraph, lam, X, epsilon: float, gamma: int = 10,
                     delta: float | None = None, mode: int = L.MODE_FAITHFUL):
         """tgtp_score_trace: scores plus the per-stage arrays (same keys as
