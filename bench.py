@@ -11,13 +11,15 @@ resident in HBM before timing starts.
 
 value = transitions/s = n * n_walks * depth / seconds per step
 (rw_experiment_saba.cpp:214-215).  One JSON line on stdout.  Further
-objects in the line: the host CPU baseline of the walks, `mt_walks` (the
+objects in the line: `saba_depth15` (the same graph at 15 hops), the host
+CPU baseline of the walks (AVX2 + OpenMP restatement), `mt_walks` (the
 MT19937 comparator walks), `tgtp` (n=4096 Chung-Lu, FAITHFUL and FAST),
-`tgtp_batch` (256 graphs, batched vs plain) and, with --precision, the
-512 x 512 rewired grid.
+`tgtp_batch` (256 graphs, batched vs plain, with the round kernel's
+roofline) and `tgtp_precision` (the 512 x 512 rewired grid on sampled
+sources).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-mt]
-                    [--no-tgtp] [--no-batch] [--precision]
+                    [--no-tgtp] [--no-batch] [--no-precision] [--no-depth15]
 """
 from __future__ import annotations
 
@@ -62,40 +64,46 @@ def traffic_for(kernel: str, workload: str):
         return None
 
 
-def bench_saba(args):
+def bench_saba(args, g=None, depth=None, steps=None, warmup=None):
+    """SABA walk passes on the n=2^20 graph with the CSR resident in HBM.
+    Returns (value, ms_per_step, roofline, config, graph)."""
     from tgtp_amd import Context, workloads as WL
 
-    depth, n_walks = args.depth, args.walks
-    t0 = time.time()
-    g = WL.chung_lu_1m()
-    log(f"[bench] graph n={g.n} m={g.m} generated in {time.time() - t0:.1f}s")
+    depth = args.depth if depth is None else depth
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
+    n_walks = args.walks
+    if g is None:
+        t0 = time.time()
+        g = WL.chung_lu_1m()
+        log(f"[bench] graph n={g.n} m={g.m} generated in {time.time() - t0:.1f}s")
     ctx = Context(0)
     dg = ctx.upload(g)
     trans = g.n * n_walks * depth  # reference definition, all n vertices
-    for i in range(args.warmup):
+    for i in range(warmup):
         _, vc, cs, st = ctx.saba_walks(dg, depth, n_walks, want_counters=False)
-        log(f"[bench] warmup {i}: {st['kernel_ms']:.1f} ms checksum={cs}")
+        log(f"[bench] depth {depth} warmup {i}: {st['kernel_ms']:.1f} ms checksum={cs}")
     walk_ms = []
     t_start = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         _, vc, cs, st = ctx.saba_walks(dg, depth, n_walks, want_counters=False)
         walk_ms.append(st["walk_ms"])
     elapsed = time.perf_counter() - t_start  # each call ends with a stream synchronize
     dg.free()
     ctx.close()
-    ms_per_step = 1e3 * elapsed / args.steps
-    value = trans / (elapsed / args.steps)
+    ms_per_step = 1e3 * elapsed / steps
+    value = trans / (elapsed / steps)
     avg_walk_s = sum(walk_ms) / len(walk_ms) / 1e3
     achieved = BYTES_PER_TRANSITION * trans / avg_walk_s / 1e9
     workload = (f"saba_walks chung-lu n=2^20 beta=2.2 m={g.m} depth={depth} "
                 f"n_walks={n_walks}")
-    log(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step, walk kernel "
+    log(f"[bench] depth {depth}, {steps} steps: {ms_per_step:.2f} ms/step, walk kernel "
         f"{avg_walk_s * 1e3:.2f} ms, {value:.3e} transitions/s, checksum={cs}")
     # one walk pass = source chunks x (saba_walk_kernel, bucket_count_kernel);
     # the unit of "launch" here is the pass, timed over all its kernels
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_for("saba_walk_pass", "cl1m_d10_w16384"),
+            "traffic": traffic_for("saba_walk_pass", f"cl1m_d{depth}_w{n_walks}"),
             "kernel": "saba_walk_kernel + bucket_count_kernel (one pass)",
             "kernel_ms": round(avg_walk_s * 1e3, 3),
             "bytes_per_unit": BYTES_PER_TRANSITION, "units_per_launch": trans}
@@ -214,14 +222,36 @@ def cpu_tgtp_baseline(graphs, eigens, eps, gpu_faithful=None, sampled=None, reps
     return out
 
 
+def reference_algorithm_sampled(graphs, eigens, samples, exact, eps):
+    """The reference algorithm's own error on the sampled edges of every
+    graph: the oracle (bit-exact to the reference build, reference tau rule)
+    runs the rows of both endpoints of each sampled edge, score = row(pos) +
+    row(reverse position) as aesc.h:532-550 sums them."""
+    import numpy as np
+
+    import oracle
+
+    threads = host_threads()
+    est = []
+    for g, (lam, X), p in zip(graphs, eigens, samples):
+        rows_of = np.repeat(np.arange(g.n, dtype=np.uint32), np.diff(g.idx.astype(np.int64)))
+        src = np.unique(np.concatenate([rows_of[p], g.adj[p]]))
+        taus = oracle.taus(g.idx, g.adj, lam, X, np.float32(eps / 2), threads=threads)
+        rows = oracle.tgtp_sources(g.idx, g.adj, taus, eps, 10, src, threads=threads)
+        est.append(rows[p] + rows[g.reverse_index()[p]])
+    return error_stats(np.concatenate(est), np.concatenate(exact), eps)
+
+
 def bench_batch(args):
     """Batched tgtp on 256 seeded Chung-Lu graphs (n log-uniform in
     [1024, 65536], exponent U(2.1, 3.0), mean degree U(8, 40)), omega=128,
     gamma=10: one autotuned, graph-captured batch run vs one plain
     tgtp_score_dev call per graph, both with graphs and eigenpairs resident
-    in HBM and scores copied back to the host.  edges/s = total CSR entries /
-    seconds.  Errors are against FP64 exact resistances of sampled edges
-    (block PCG, tgtp_amd.exact_sc)."""
+    in HBM and scores copied back to the host, each side the minimum of 3
+    timed runs (the maximum is reported too).  edges/s = total CSR entries /
+    seconds.  Errors are against FP64 exact resistances of 32 sampled edges
+    per graph (block PCG, tgtp_amd.exact_sc), beside the reference
+    algorithm's own error on the same edges."""
     import numpy as np
     import torch
 
@@ -251,29 +281,35 @@ def bench_batch(args):
     out = {"workload": "256 Chung-Lu graphs " + WL.describe(
                {k: v for k, v in WL.BATCH256.items() if k != "count"}),
            "graphs": len(graphs), "sum_n": sum(g.n for g in graphs), "sum_m": total_m,
-           "omega": 128, "gamma": 10, "eigen_prep_s": round(t_eig, 1)}
-    reps = 2
+           "omega": 128, "gamma": 10, "eigen_prep_s": round(t_eig, 1),
+           "tau_rule": "bounded (FAST default; the reference's rule leaves eps on bridge edges)"}
+    reps = 3
     fast_out = None
+    roof = None
     with Context(0) as ctx:
-        for eps in (0.05, 0.01):
+        modes = [(m, nm) for m, nm in ((MODE_FAST_FP32, "fast_fp32"), (MODE_FAST_BF16, "fast_bf16"))
+                 if nm[5:] in args.batch_modes.split(",")]
+        for eps in [float(x) for x in args.batch_eps.split(",")]:
             leg = {}
-            for mode, name in ((MODE_FAST_FP32, "fast_fp32"), (MODE_FAST_BF16, "fast_bf16")):
-                plain_s, plain_out, plain_each = 0.0, [], []
-                for gi, (g, e) in enumerate(zip(graphs, eig)):
+            for mode, name in modes:
+                plain_min, plain_max, plain_out, plain_each = 0.0, 0.0, [], []
+                for gi, (g, e) in enumerate(zip(graphs, eig) if args.batch_plain else []):
                     if gi % 64 == 63:
                         log(f"[bench] batch eps={eps} {name}: plain {gi + 1}/{len(graphs)}")
                     dg = ctx.upload(g)
                     ctx.score(dg, *e, eps, 10, mode=mode)  # eigen upload + plan
-                    best = None
-                    for _ in range(1):
+                    ts = []
+                    for _ in range(reps):
                         t1 = time.perf_counter()
                         s, _ = ctx.score(dg, *e, eps, 10, mode=mode)
-                        dt = time.perf_counter() - t1
-                        best = dt if best is None else min(best, dt)
-                    plain_s += best
-                    plain_each.append(best)
+                        ts.append(time.perf_counter() - t1)
+                    plain_min += min(ts)
+                    plain_max += max(ts)
+                    plain_each.append(min(ts))
                     plain_out.append(s)
                     dg.free()
+                if not args.batch_plain:
+                    plain_min = plain_max = float("nan")
                 t1 = time.perf_counter()
                 b = ctx.batch(graphs, eig, eps, 10, mode=mode, autotune=True, graph=True)
                 t_create = time.perf_counter() - t1
@@ -285,26 +321,45 @@ def bench_batch(args):
                     times.append(time.perf_counter() - t1)
                     kms.append(st["kernel_ms"])
                 cfg = b.config()
+                if mode == MODE_FAST_FP32 and eps == 0.01:
+                    prof = b.profile()
+                    avg_ms = prof["round_ms"] / max(1, prof["round_launches"])
+                    per_launch = prof["round_bytes"] / max(1, prof["round_launches"])
+                    achieved = prof["round_bytes"] / (prof["round_ms"] / 1e3) / 1e9
+                    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                            "traffic": traffic_for("blk_round4_kernel", "batch256_eps0.01"),
+                            "kernel": "blk_round4_kernel<16,4> (diffusion round, FAST_FP32)",
+                            "kernel_ms": round(avg_ms, 4), "launches": prof["round_launches"],
+                            "bytes_per_launch": round(per_launch),
+                            "work_unit": "per source-round: 4 B per edge + 4 B per vertex of its "
+                                         "graph; per block-round: 4 B per edge + 8 B per vertex",
+                            "profile": prof}
+                    log(f"[bench] batch eps=0.01 fast_fp32 round kernel: {prof['round_launches']} "
+                        f"launches, {avg_ms:.3f} ms avg, {achieved:.0f} GB/s algorithmic")
                 b.close()
                 if mode == MODE_FAST_FP32:
                     fast_out = outs
-                best = min(times)
-                bit_equal = all(np.array_equal(a.view(np.uint32), p.view(np.uint32))
-                                for a, p in zip(outs, plain_out))
+                best, worst = min(times), max(times)
+                bit_equal = (all(np.array_equal(a.view(np.uint32), p.view(np.uint32))
+                                 for a, p in zip(outs, plain_out)) if args.batch_plain else None)
                 est = np.concatenate([o[p] for o, p in zip(outs, samples)])
                 leg[name] = {"batched_edges_per_s": total_m / best, "batched_seconds": best,
-                             "batched_kernel_ms": min(kms),
-                             "plain_edges_per_s": total_m / plain_s, "plain_seconds": plain_s,
-                             "batched_over_plain": plain_s / best,
+                             "batched_seconds_max": worst, "batched_kernel_ms": min(kms),
+                             "plain_edges_per_s": total_m / plain_min,
+                             "plain_seconds": plain_min, "plain_seconds_max": plain_max,
+                             "batched_over_plain": plain_min / best,
+                             "batched_over_plain_range": [plain_min / worst, plain_max / best],
+                             "timing": f"min (and max) of {reps} runs per side",
                              "create_autotune_capture_s": round(t_create, 2),
                              "autotuned": cfg, "bitwise_equal_plain": bit_equal,
                              "vs_exact_fp64_sampled": error_stats(est, np.concatenate(exact),
                                                                   eps)}
                 log(f"[bench] batch eps={eps} {name}: batched {total_m / best:.3e} edges/s "
-                    f"({best * 1e3:.1f} ms), plain {total_m / plain_s:.3e} "
-                    f"({plain_s * 1e3:.1f} ms), x{plain_s / best:.2f}, cfg {cfg}, "
-                    f"max err {leg[name]['vs_exact_fp64_sampled']['max_abs']:.4f}")
-                if mode == MODE_FAST_FP32:
+                    f"({best * 1e3:.1f}-{worst * 1e3:.1f} ms), plain {total_m / plain_min:.3e} "
+                    f"({plain_min * 1e3:.1f}-{plain_max * 1e3:.1f} ms), x{plain_min / best:.3f}, "
+                    f"cfg {cfg}, max err {leg[name]['vs_exact_fp64_sampled']['max_abs']:.4f}")
+                if mode == MODE_FAST_FP32 and args.batch_plain:
                     # the batch's small graphs alone, where per-call costs dominate
                     small = [i for i, g in enumerate(graphs) if g.n <= args.batch_small_n]
                     if small:
@@ -323,93 +378,179 @@ def bench_batch(args):
                             f"{args.batch_small_n}: batched {sm / sb:.3e} edges/s, plain "
                             f"{sm / sp:.3e}, x{sp / sb:.2f}")
             if not args.no_cpu:
+                t1 = time.time()
+                leg["reference_algorithm_vs_exact_fp64_sampled"] = reference_algorithm_sampled(
+                    graphs, eig, samples, exact, eps)
+                log(f"[bench] batch eps={eps}: reference algorithm on every sampled edge, max err "
+                    f"{leg['reference_algorithm_vs_exact_fp64_sampled']['max_abs']:.4f} "
+                    f"({time.time() - t1:.1f}s)")
                 every = max(1, len(graphs) // args.batch_cpu_graphs)
                 sub = list(range(0, len(graphs), every))
                 leg["cpu_baseline"] = cpu_tgtp_baseline(
                     [graphs[i] for i in sub], [eig[i] for i in sub], eps,
-                    sampled=[(samples[i], exact[i]) for i in sub], reps=1)
+                    sampled=[(samples[i], exact[i]) for i in sub], reps=reps)
                 leg["cpu_baseline"]["sample"] += f"; every {every}th graph of the batch"
-                fe = np.concatenate([fast_out[i][samples[i]] for i in sub])
-                leg["fast_fp32_vs_exact_fp64_same_graphs"] = error_stats(
-                    fe, np.concatenate([exact[i] for i in sub]), eps)
-                log(f"[bench] batch eps={eps} cpu {leg['cpu_baseline']['edges_per_s']:.3e} edges/s; "
-                    f"max err on the cpu graphs: reference algorithm "
-                    f"{leg['cpu_baseline']['reference_algorithm_vs_exact_fp64_sampled']['max_abs']:.4f}, "
-                    f"fast_fp32 {leg['fast_fp32_vs_exact_fp64_same_graphs']['max_abs']:.4f}")
+                if fast_out is not None:
+                    fe = np.concatenate([fast_out[i][samples[i]] for i in sub])
+                    leg["fast_fp32_vs_exact_fp64_same_graphs"] = error_stats(
+                        fe, np.concatenate([exact[i] for i in sub]), eps)
+                log(f"[bench] batch eps={eps} cpu {leg['cpu_baseline']['edges_per_s']:.3e} edges/s")
             out[f"eps_{eps}"] = leg
+    if roof is not None:
+        out["roofline"] = roof
     return out
 
 
 def bench_precision(args):
     """Precision form of tgtp: the 512 x 512 grid with 5% of edges rewired
-    (n = 262,144), omega = 128, gamma = 10, eps 0.01 and 0.005, FAST_FP32
-    and FAST_BF16 (projection in FP32 / BF16 with FP32 accumulation, FP64
-    score sums); max / mean relative error against FP64 exact resistances of
-    sampled edges (block PCG).  No CPU leg: the reference algorithm needs
-    ~2.4e13 pushed edges here (hours on the host)."""
+    (n = 262,144, m = 1,046,528), omega = 128, gamma = 10, eps 0.01 and
+    0.005, FAST_FP32 and FAST_BF16 (projection in FP32 / BF16 with FP32
+    accumulation, FP64 score sums).  A full run takes ~3 minutes per leg, so
+    a seeded uniform sample of source ranges (4 x 64 sources) runs through
+    tgtp_score_sources; per-source work is independent (aesc.h:456-527), so
+    sources/s extrapolates to the whole graph.  Errors: max / mean relative
+    error of the edges whose endpoints are both sampled against FP64 exact
+    resistances (block PCG).  CPU leg: the oracle (the reference algorithm)
+    on the same sources, eps 0.01.  --precision-full adds full-graph runs."""
     import numpy as np
     import torch
 
+    import oracle
     from tgtp_amd import MODE_FAST_BF16, MODE_FAST_FP32, Context, eigenprep as E, exact_sc
     from tgtp_amd import workloads as WL
 
-    os.environ["TGTP_PROGRESS"] = "1"  # calls of minutes: the library reports each subset
     side = args.precision_side
     g = WL.G.grid_rewired(side, side, 0.05, seed=1)
     t0 = time.time()
-    lam, X = E.eigens_f32(*E.compute_eigens_many(
-        [g], 128, device="cuda:0",
-        on_cycle=lambda c, r: log(f"[bench] precision eigen cycle {c}: residual {r:.2e} "
-                                  f"({time.time() - t0:.0f}s)") if c % 5 == 0 else None)[0])
+    lam, X = E.eigens_f32(*E.compute_eigens_many([g], 128, device="cuda:0")[0])
     t_eig = time.time() - t0
-    rng = np.random.default_rng(3)
-    pos = np.sort(rng.choice(g.m, size=min(args.precision_sample_edges, g.m), replace=False))
-    R = exact_sc.sampled_scores(g, pos, "cuda:0")
+    rng = np.random.default_rng(11)
+    n_rng = max(1, args.precision_sources // 64)
+    starts = np.sort(rng.choice(g.n // 64, size=n_rng, replace=False)) * 64
+    ranges = [(int(x), int(x) + 64) for x in starts]
+    mask = np.zeros(g.n, dtype=bool)
+    for b, e in ranges:
+        mask[b:e] = True
+    rows_of = np.repeat(np.arange(g.n), np.diff(g.idx.astype(np.int64)))
+    both = np.nonzero(mask[rows_of] & mask[g.adj])[0]
+    R = exact_sc.sampled_scores(g, both, "cuda:0")
     torch.cuda.empty_cache()
-    log(f"[bench] precision grid {side}x{side}: n={g.n} m={g.m}, eigenpairs {t_eig:.1f}s")
+    n_src = int(mask.sum())
+    log(f"[bench] precision grid {side}x{side}: n={g.n} m={g.m}, eigenpairs {t_eig:.1f}s, "
+        f"{n_src} sampled sources, {both.shape[0]} edges with both ends sampled")
     out = {"workload": f"grid {side}x{side}, 5% rewired, seed 1", "n": g.n, "m": g.m,
-           "omega": 128, "gamma": 10, "eigen_prep_s": round(t_eig, 1)}
+           "omega": 128, "gamma": 10, "eigen_prep_s": round(t_eig, 1),
+           "sample": f"{n_rng} seeded ranges of 64 sources ({n_src} sources), "
+                     f"{both.shape[0]} edges with both endpoints sampled",
+           "source_ranges": ranges}
     with Context(0) as ctx:
         dg = ctx.upload(g)
         for eps in (0.01, 0.005):
             leg = {}
             for mode, name in ((MODE_FAST_FP32, "fast_fp32"), (MODE_FAST_BF16, "fast_bf16")):
-                t1 = time.perf_counter()
-                s, st = ctx.score(dg, lam, X, eps, 10, mode=mode)
-                dt = time.perf_counter() - t1
-                leg[name] = {"edges_per_s": g.m / dt, "seconds": dt,
-                             "taus_ms": st["taus_ms"], "diffuse_ms": st["diffuse_ms"],
-                             "vs_exact_fp64_sampled": error_stats(s, R, eps, pos)}
+                ctx.score_sources(dg, lam, X, eps, ranges, mode=mode)
+                best, sc = None, None
+                for _ in range(3):
+                    t1 = time.perf_counter()
+                    _, sc, st = ctx.score_sources(dg, lam, X, eps, ranges, mode=mode)
+                    dt = time.perf_counter() - t1
+                    best = dt if best is None else min(best, dt)
+                err = np.abs(sc[both].astype(np.float64) - R)
+                leg[name] = {"sources_per_s": n_src / best, "seconds": best,
+                             "kernel_ms": st["kernel_ms"], "taus_ms": st["taus_ms"],
+                             "extrapolated_full_s": g.n / (n_src / best),
+                             "vs_exact_fp64_sampled": error_stats(sc, R, eps, both)}
                 e = leg[name]["vs_exact_fp64_sampled"]
-                log(f"[bench] precision eps={eps} {name}: {dt:.1f}s, max rel {e['max_rel']:.4f} "
-                    f"mean rel {e['mean_rel']:.5f} max abs {e['max_abs']:.4f}")
+                log(f"[bench] precision eps={eps} {name}: {best * 1e3:.0f} ms for {n_src} sources, "
+                    f"max rel {e['max_rel']:.2e} mean rel {e['mean_rel']:.2e} max abs "
+                    f"{e['max_abs']:.2e}")
+                if args.precision_full:
+                    t1 = time.perf_counter()
+                    s_full, _ = ctx.score(dg, lam, X, eps, 10, mode=mode)
+                    dtf = time.perf_counter() - t1
+                    leg[name]["full_graph"] = {"edges_per_s": g.m / dtf, "seconds": dtf,
+                                               "vs_exact_fp64_sampled": error_stats(s_full, R,
+                                                                                    eps, both)}
+            if eps == 0.01 and not args.no_cpu:
+                src = np.nonzero(mask)[0].astype(np.uint32)
+                threads = host_threads()
+                taus = oracle.taus(g.idx, g.adj, lam, X, np.float32(eps / 2), threads=threads)
+                t1 = time.perf_counter()
+                oracle.tgtp_sources(g.idx, g.adj, taus, eps, 10, src, threads=threads)
+                dtc = time.perf_counter() - t1
+                leg["cpu_baseline"] = {"sources_per_s": n_src / dtc, "seconds": dtc,
+                                       "extrapolated_full_s": g.n / (n_src / dtc),
+                                       "cores": threads, "kind": "port",
+                                       "sample": f"the same {n_src} sources (oracle C++ OpenMP, "
+                                                 "schedule(dynamic) over sources)"}
+                log(f"[bench] precision eps={eps} cpu: {n_src / dtc:.1f} sources/s on {threads} "
+                    f"threads (full graph ~{g.n / (n_src / dtc):.0f} s)")
             out[f"eps_{eps}"] = leg
         dg.free()
     return out
 
 
-def cpu_saba_baseline(g, depth, n_walks, sample_sources: int):
-    """The oracle's OpenMP walk (schedule(dynamic) over sources, per-thread
-    counters) on evenly spread source slices of the same workload."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_saba_baseline(g, depth, n_walks):
+    """The AVX2 + OpenMP SABA restatement of the oracle (bouquets of 8 walkers
+    in one AVX2 register, per-thread counters, equal to the scalar oracle bit
+    for bit) on evenly spread source slices of the same workload: one thread,
+    then all host threads with the reference's schedule(dynamic, 1024)
+    (rw_experiment_saba.cpp:194) and with a chunk of 16.  value = the best."""
+    import numpy as np
+
     import oracle
 
     threads = host_threads()
-    slices = 32
-    per = max(1, sample_sources // slices)
-    step = g.n // slices
-    oracle.saba_walks(g.idx, g.adj, depth, n_walks, 0, min(g.n, 64), threads=threads, chunk=8)
-    t0 = time.perf_counter()
-    total = 0
-    for k in range(slices):
-        lo = k * step
-        hi = min(g.n, lo + per)
-        oracle.saba_walks(g.idx, g.adj, depth, n_walks, lo, hi, threads=threads, chunk=8)
-        total += (hi - lo) * n_walks * depth
-    dt = time.perf_counter() - t0
-    return {"value": total / dt, "unit": "transitions/s", "cores": threads, "kind": "port",
-            "sample": f"{slices} slices x {per} sources (of {g.n}), depth {depth}, "
-                      f"{n_walks} walks/vertex = {total:.3e} transitions in {dt:.1f}s; "
-                      f"oracle C++ OpenMP dynamic chunk 8, uint32 counters"}
+    step = g.n // 32
+
+    def timed(per, th, chunk, slices=32):
+        # evenly spread slices of `per` consecutive sources
+        t0 = time.perf_counter()
+        total = 0
+        stride = g.n // slices
+        for k in range(slices):
+            lo = k * stride
+            hi = min(g.n, lo + per)
+            oracle.saba_walks_avx2(g.idx, g.adj, depth, n_walks, lo, hi, threads=th, chunk=chunk)
+            total += (hi - lo) * n_walks * depth
+        dt = time.perf_counter() - t0
+        return total / dt, total, dt
+
+    # a source slice checked against the scalar oracle first
+    a = oracle.saba_walks_avx2(g.idx, g.adj, depth, n_walks, 0, 64, threads=threads)
+    b = oracle.saba_walks(g.idx, g.adj, depth, n_walks, 0, 64, threads=threads)
+    legs = {}
+    v1, t1, d1 = timed(16, 1, 1024)
+    legs["avx2_1thread"] = {"transitions_per_s": v1, "sources": 32 * 16, "seconds": round(d1, 2)}
+    # the reference's chunk of 1024 sources needs slices of many chunks to
+    # keep every thread busy: 2 slices of 16 chunks (the same 32,768 sources)
+    vr, tr, dr = timed(16384, threads, 1024, slices=2)
+    legs[f"avx2_{threads}threads_dynamic1024"] = {"transitions_per_s": vr, "sources": 2 * 16384,
+                                                  "seconds": round(dr, 2)}
+    vt, tt, dt = timed(1024, threads, 16)
+    legs[f"avx2_{threads}threads_dynamic16"] = {"transitions_per_s": vt, "sources": 32 * 1024,
+                                                "seconds": round(dt, 2)}
+    best = max(vr, vt)
+    log(f"[bench] cpu walks (AVX2): 1 thread {v1:.3e}, {threads} threads {vr:.3e} (chunk 1024) / "
+        f"{vt:.3e} (chunk 16)")
+    return {"value": best, "unit": "transitions/s", "cores": threads, "kind": "port",
+            "sample": f"32,768 of {g.n} sources (2 slices of 16,384 for dynamic,1024; 32 "
+                      f"evenly spread slices of 1,024 for dynamic,16), depth {depth}, {n_walks} "
+                      f"walks/vertex = {tt:.3e} transitions; AVX2 + OpenMP restatement of the "
+                      f"oracle (orc_saba_walks_avx2), best of the two schedules on {threads} "
+                      f"threads",
+            "cpu_model": cpu_model(), "compiler": "g++ -std=c++20 -O2 -fopenmp, AVX2 target",
+            "equal_scalar_oracle": bool(np.array_equal(a, b)), "legs": legs}
 
 
 def bench_mt(args, g_big=None):
@@ -472,20 +613,26 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--depth", type=int, default=10)
     ap.add_argument("--walks", type=int, default=16384)
-    ap.add_argument("--cpu-sample-sources", type=int, default=32768)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tgtp", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-saba", action="store_true", help="tgtp legs only (profiling)")
     ap.add_argument("--no-mt", action="store_true", help="skip the MT19937 comparator walks")
-    ap.add_argument("--precision", action="store_true",
-                    help="add the 512x512 rewired-grid precision leg (minutes)")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the 512x512 rewired-grid precision leg (sampled sources)")
+    ap.add_argument("--precision-full", action="store_true",
+                    help="also score the whole 512x512 grid in each precision leg (minutes)")
     ap.add_argument("--precision-side", type=int, default=512)
-    ap.add_argument("--precision-sample-edges", type=int, default=256)
+    ap.add_argument("--precision-sources", type=int, default=256)
+    ap.add_argument("--no-depth15", action="store_true", help="skip the 15-hop walk leg")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-sample-edges", type=int, default=32)
     ap.add_argument("--batch-cpu-graphs", type=int, default=16)
     ap.add_argument("--batch-small-n", type=int, default=8192)
+    ap.add_argument("--batch-eps", default="0.05,0.01", help="comma-separated eps of the batch legs")
+    ap.add_argument("--batch-modes", default="fp32,bf16", help="batch modes: fp32, bf16")
+    ap.add_argument("--batch-plain", type=int, default=1,
+                    help="0: skip the per-graph plain runs (profiling the batched run alone)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
@@ -498,7 +645,7 @@ def main() -> int:
             line["tgtp"] = bench_tgtp(args)
         if not args.no_batch:
             line["tgtp_batch"] = bench_batch(args)
-        if args.precision:
+        if not args.no_precision:
             line["tgtp_precision"] = bench_precision(args)
         print(json.dumps(line), flush=True)
         return 0
@@ -508,10 +655,13 @@ def main() -> int:
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
             "data": "synthetic (seeded Chung-Lu generator, regenerated at run time)",
             "config": cfg, "roofline": roof}
+    if not args.no_depth15:
+        v15, ms15, roof15, cfg15, _ = bench_saba(args, g=g, depth=15, steps=3, warmup=1)
+        line["saba_depth15"] = {"value": v15, "unit": "transitions/s", "ms_per_step": ms15,
+                                "steps": 3, "warmup": 1, "config": cfg15, "roofline": roof15}
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
-        line["cpu_baseline"] = cpu_saba_baseline(g, args.depth, args.walks,
-                                                 args.cpu_sample_sources)
+        line["cpu_baseline"] = cpu_saba_baseline(g, args.depth, args.walks)
         line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
     if not args.no_mt:
         line["mt_walks"] = bench_mt(args, g)
@@ -520,7 +670,7 @@ def main() -> int:
         line["tgtp"] = bench_tgtp(args)
     if not args.no_batch:
         line["tgtp_batch"] = bench_batch(args)
-    if args.precision:
+    if not args.no_precision:
         line["tgtp_precision"] = bench_precision(args)
     print(json.dumps(line), flush=True)
     return 0

@@ -198,6 +198,27 @@ void tgtp_batch_destroy(tgtp_ctx* ctx, tgtp_batch* batch);
 /* Launch shape the batch runs with (after autotuning, if requested). */
 int tgtp_batch_config(const tgtp_batch* batch, uint32_t* light_degree, uint32_t* small_bouquets,
                       uint32_t* slots, uint32_t* max_slots);
+/* Diagnostic run of a FAST batch (uncaptured, its tuned launch shape) with
+ * an event pair around every launch of the diffusion round kernel, the stop
+ * kernel and the post kernel, on the stream each runs on.  round_bytes are
+ * the algorithmic bytes of all round launches: per source and round after
+ * the first, 4 bytes per edge of its graph (its value gathered once per
+ * neighbour entry) and 4 per vertex (its new value written); per block of 64
+ * sources and round, 4 bytes per edge (neighbour ids) and 8 per vertex (row
+ * descriptors).  No counterpart in the reference. */
+typedef struct tgtp_profile {
+  double total_ms;          /* the whole run, event to event */
+  double taus_ms;           /* tau projection */
+  double round_ms;          /* sum over round-kernel launches */
+  uint64_t round_launches;
+  uint64_t round_bytes;
+  double stop_ms;
+  uint64_t stop_launches;
+  double post_ms;
+  uint64_t post_launches;
+  uint64_t source_rounds;   /* sum over sources of the rounds each took */
+} tgtp_profile;
+int tgtp_batch_profile(tgtp_ctx* ctx, tgtp_batch* batch, tgtp_profile* out);
 /* One-shot: create (no tuning, no capture), run, destroy. */
 int tgtp_score_batch(tgtp_ctx* ctx, int32_t count, const tgtp_graph* graphs,
                      const tgtp_eigens* eigens, const tgtp_params* params,

@@ -43,6 +43,8 @@ def lib() -> C.CDLL:
         L.orc_splitmix32.restype = C.c_uint32
         L.orc_saba_walks.argtypes = [C.c_uint64, u32p, u32p, C.c_uint32, C.c_uint32, C.c_uint32,
                                      C.c_uint64, u32p, C.c_int, C.c_int]
+        L.orc_saba_walks_avx2.argtypes = [C.c_uint64, u32p, u32p, C.c_uint32, C.c_uint32,
+                                          C.c_uint32, C.c_uint64, u32p, C.c_int, C.c_int]
         L.orc_mt_walks.argtypes = [C.c_uint64, u32p, u32p, C.c_uint32, C.c_uint32, C.c_uint32,
                                    C.c_uint64, C.c_int, u32p, C.c_int]
         L.orc_mt19937_nth.argtypes = [C.c_uint32, C.c_uint64]
@@ -101,6 +103,23 @@ def saba_walks(idx, adj, depth: int, n_walks: int, src_begin: int = 0, src_end=N
                               depth, n_walks, _p(cnt, C.c_uint32), threads, chunk)
     if rc != 0:
         raise ValueError("n_walks must be a multiple of 8 within the 32-bit grid domain")
+    return cnt
+
+
+def saba_walks_avx2(idx, adj, depth: int, n_walks: int, src_begin: int = 0, src_end=None,
+                    threads: int = 0, chunk: int = 1024) -> np.ndarray:
+    """saba_walks with AVX2 bouquets (the host CPU baseline); raises
+    RuntimeError where the CPU has no AVX2."""
+    idx, adj = _u32(idx), _u32(adj)
+    n = idx.shape[0] - 1
+    src_end = n if src_end is None else src_end
+    cnt = np.zeros(n, dtype=np.uint32)
+    rc = lib().orc_saba_walks_avx2(n, _p(idx, C.c_uint32), _p(adj, C.c_uint32), src_begin,
+                                   src_end, depth, n_walks, _p(cnt, C.c_uint32), threads, chunk)
+    if rc == -2:
+        raise RuntimeError("this CPU has no AVX2")
+    if rc != 0:
+        raise ValueError("n_walks must be a positive multiple of 8; graph below 2^31 entries")
     return cnt
 
 

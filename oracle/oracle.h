@@ -13,6 +13,8 @@
  * bit for bit.
  *
  * Pinning (how each function is checked against something other than itself):
+ *   orc_saba_walks_avx2: equal to orc_saba_walks on every test graph,
+ *       including the fallback-forcing known answers and a non-symmetric CSR.
  *   orc_fmix32 / orc_splitmix32 / orc_saba_walks:
  *       recorded known answers of the reference's own saba_randomwalk
  *       (C9 at 40 hops and K5 at 20 hops, both forcing the whole-bouquet
@@ -67,6 +69,14 @@ int orc_saba_walks(uint64_t n, const uint32_t* idx, const uint32_t* adj,
                    uint32_t src_begin, uint32_t src_end, uint32_t depth,
                    uint64_t n_walks, uint32_t* counters, int threads,
                    int chunk);
+
+/* The same walks with the 8 walkers of a bouquet in one AVX2 register
+ * (orc_walk_avx2.cpp): equal counters, the host CPU baseline of the walk
+ * engine.  Returns -2 where the CPU lacks AVX2, -1 for n_walks not a
+ * positive multiple of 8 or a graph beyond 2^31 vertices / entries. */
+int orc_saba_walks_avx2(uint64_t n, const uint32_t* idx, const uint32_t* adj,
+                        uint32_t src_begin, uint32_t src_end, uint32_t depth,
+                        uint64_t n_walks, uint32_t* counters, int threads, int chunk);
 
 /* One source's walks into a caller counter array (single thread). */
 void orc_saba_walk_source(const uint32_t* idx, const uint32_t* adj,

@@ -72,6 +72,32 @@ def test_captured_graph_replays(gpu_ctx, small_batch):
     b.close()
 
 
+@pytest.mark.parametrize("streams", ["1", "2"])
+def test_captured_replays_on_several_subsets(gpu_ctx, small_batch, monkeypatch, streams):
+    """Small state subsets (1 MiB) split the batch into many subsets: on two
+    streams the capture is one graph per stream share, replayed with the
+    fork / join events between them.  Every replay, with host work and other
+    launches in between, equals the uncaptured run bit for bit."""
+    graphs, eig = small_batch
+    monkeypatch.setenv("TGTP_FAST_STATE_MIB", "1")
+    monkeypatch.setenv("TGTP_BATCH_STREAMS", streams)
+    plain = gpu_ctx.batch(graphs, eig, 0.05, 10, mode=MODE_FAST_FP32)
+    ref, _ = plain.run()
+    plain.close()
+    b = gpu_ctx.batch(graphs, eig, 0.05, 10, mode=MODE_FAST_FP32, graph=True)
+    for rep in range(3):
+        outs, st = b.run()
+        for k, (a, r) in enumerate(zip(outs, ref)):
+            assert np.array_equal(a.view(np.uint32), r.view(np.uint32)), f"replay {rep} graph {k}"
+        gpu_ctx.score(graphs[1], *eig[1], 0.05, 10, mode=MODE_FAST_FP32)  # interleaved work
+        _say(f"  streams={streams} replay {rep}: {st['kernel_ms']:.2f} ms")
+    b.close()
+    # and equal to the per-graph calls
+    for k, (g, e) in enumerate(zip(graphs, eig)):
+        s, _ = gpu_ctx.score(g, *e, 0.05, 10, mode=MODE_FAST_FP32)
+        assert np.array_equal(s.view(np.uint32), ref[k].view(np.uint32)), f"graph {k}"
+
+
 def test_autotuned_batch_accuracy(gpu_ctx, small_batch):
     graphs, eig = small_batch
     b = gpu_ctx.batch(graphs, eig, 0.05, 10, mode=MODE_FAST_FP32, autotune=True, graph=True)

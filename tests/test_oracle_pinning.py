@@ -249,3 +249,28 @@ def test_saba_walker_stops_at_vertex_without_neighbours(depth, n_walks, falls, d
     assert np.array_equal(got, exp)
     # the rules each case exercises
     assert stopped > 0 and (fell > 0) == falls and (dz > 0) == dead_zero
+
+
+@pytest.mark.parametrize("case", ["c9", "k5", "sink", "cl", "grid"])
+def test_avx2_walks_equal_scalar(case):
+    """The AVX2 host baseline gives the scalar oracle's counters bit for bit,
+    also where the whole-bouquet fallback runs (C9, K5), where walkers stop
+    at a vertex without neighbours (non-symmetric CSR) and on larger graphs,
+    on several threads and chunk sizes."""
+    if case == "c9":
+        g, depth, W = G.cycle(9), 40, 64
+    elif case == "k5":
+        g, depth, W = G.complete(5), 20, 64
+    elif case == "sink":
+        g, depth, W = G.Graph(SABA_SINK_IDX, SABA_SINK_ADJ), 40, 64
+    elif case == "cl":
+        g, depth, W = G.chung_lu(3000, 2.3, 14.0, seed=4), 15, 256
+    else:
+        g, depth, W = G.grid_rewired(40, 40, 0.05, seed=2), 30, 128
+    ref = oracle.saba_walks(g.idx, g.adj, depth, W)
+    try:
+        for threads, chunk in ((1, 1024), (4, 7), (8, 1)):
+            got = oracle.saba_walks_avx2(g.idx, g.adj, depth, W, threads=threads, chunk=chunk)
+            assert np.array_equal(got, ref), f"{case} threads={threads} chunk={chunk}"
+    except RuntimeError:
+        pytest.skip("no AVX2 on this host")

@@ -69,12 +69,24 @@ class StatsStruct(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class ProfileStruct(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("taus_ms", C.c_double), ("round_ms", C.c_double),
+                ("round_launches", C.c_uint64), ("round_bytes", C.c_uint64),
+                ("stop_ms", C.c_double), ("stop_launches", C.c_uint64),
+                ("post_ms", C.c_double), ("post_launches", C.c_uint64),
+                ("source_rounds", C.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 # every symbol the two public headers declare
 EXPORTED = [
     "tgtp_ctx_create", "tgtp_ctx_destroy", "tgtp_ctx_last_error", "tgtp_strerror",
     "tgtp_graph_upload", "tgtp_graph_free", "tgtp_saba_walks", "tgtp_mt_walks", "tgtp_taus",
     "tgtp_score", "tgtp_score_dev", "tgtp_score_trace", "tgtp_score_sources",
     "tgtp_batch_create", "tgtp_batch_run", "tgtp_batch_destroy", "tgtp_batch_config",
+    "tgtp_batch_profile",
     "tgtp_score_batch",
     "tgtp_load_txt", "tgtp_csr_free", "tgtp_save_csr_bin", "tgtp_load_csr_bin",
     "tgtp_load_eigens_txt", "tgtp_save_eigens_bin", "tgtp_load_eigens_bin",
@@ -137,6 +149,7 @@ def lib() -> C.CDLL:
     L.tgtp_batch_run.argtypes = [vp, vp, P(P(C.c_float)), P(StatsStruct)]
     L.tgtp_batch_destroy.argtypes = [vp, vp]
     L.tgtp_batch_destroy.restype = None
+    L.tgtp_batch_profile.argtypes = [vp, vp, P(ProfileStruct)]
     L.tgtp_batch_config.argtypes = [vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32),
                                     P(C.c_uint32)]
     L.tgtp_score_batch.argtypes = [vp, C.c_int32, P(GraphStruct), P(EigensStruct),

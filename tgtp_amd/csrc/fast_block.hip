@@ -1123,9 +1123,35 @@ int block_post_per_cu(uint32_t stage) {
   return std::max(1, per_cu);
 }
 
+int LaunchTimer::begin(hipStream_t s, int k) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (hipEventCreate(&a) != hipSuccess) return -1;
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return -1;
+  }
+  ev.push_back(a);
+  ev.push_back(b);
+  kind.push_back(k);
+  (void)hipEventRecord(a, s);
+  return static_cast<int>(kind.size()) - 1;
+}
+
+void LaunchTimer::end(hipStream_t s, int pair) {
+  if (pair >= 0) (void)hipEventRecord(ev[2 * pair + 1], s);
+}
+
+LaunchTimer::~LaunchTimer() {
+  for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+}
+
 int launch_block_subset(tgtp_ctx* ctx, hipStream_t s, const BlockArgs& A, const BlockSubset& S,
                         const uint32_t* round_tiles, const uint32_t* round_hubs,
-                        uint32_t post_slots) {
+                        uint32_t post_slots, LaunchTimer* timer) {
+  auto tb = [&](int k) { return timer ? timer->begin(s, k) : -1; };
+  auto te = [&](int pair) {
+    if (timer) timer->end(s, pair);
+  };
   const uint32_t nb = S.b1 - S.b0;
   const uint32_t col_blocks = (nb * kBlkCols + 3) / 4;
   const uint64_t zb = std::min<uint64_t>(16384, (S.floats + 255) / 256 + 1);
@@ -1143,6 +1169,7 @@ int launch_block_subset(tgtp_ctx* ctx, hipStream_t s, const BlockArgs& A, const 
     const uint32_t tr = round_tiles[r];
     if (tr == 0) continue;
     const uint32_t hr = round_hubs[r], hub_wgs = (hr + 7u) & ~7u;
+    const int tr_pair = tb(LaunchTimer::kRound);
     if (A.round_form == 1) {
       blk_round4_kernel<8, 5><<<grid_for(tr), 256, 0, s>>>(A, S.b0, S.b1, tr, 0u, cur, nxt);
       if (hr) blk_hub_kernel<<<hr, 256, 0, s>>>(A, S.b0, S.b1, hr, cur, nxt);
@@ -1153,15 +1180,20 @@ int launch_block_subset(tgtp_ctx* ctx, hipStream_t s, const BlockArgs& A, const 
       blk_round_kernel<<<grid_for(tr), 256, 0, s>>>(A, S.b0, S.b1, tr, cur, nxt);
       if (hr) blk_hub_kernel<<<hr, 256, 0, s>>>(A, S.b0, S.b1, hr, cur, nxt);
     }
+    te(tr_pair);
+    const int st_pair = tb(LaunchTimer::kStop);
     blk_stop_kernel<<<col_blocks, 256, 0, s>>>(A, S.b0, S.b1, nxt);
+    te(st_pair);
   }
   blk_transpose_kernel<<<tile_grid, 256, 0, s>>>(A, S.b0, S.b1, S.tiles);
   blk_zero_kernel<<<1, 256, 0, s>>>(reinterpret_cast<float*>(A.next_src), 1);
   // the LDS buffer holds the subset's largest staged vector (smaller
   // buffers, more workgroups per CU; none when every vector is read in place)
+  const int po_pair = tb(LaunchTimer::kPost);
   blk_post_kernel<<<std::max(1u, std::min(post_slots, nb * kBlkCols)), kPB,
                     static_cast<size_t>(S.post_stage) * sizeof(float), s>>>(A, S.b0, S.b1,
                                                                            S.post_stage);
+  te(po_pair);
   TGTP_HIP_CHECK(ctx, hipGetLastError());
   return TGTP_OK;
 }

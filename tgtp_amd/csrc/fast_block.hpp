@@ -16,6 +16,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #include "common.hpp"
 
@@ -90,6 +91,18 @@ struct BlockSubset {
   uint32_t post_per_cu;  // post workgroups resident per CU with that LDS
 };
 
+// Optional per-kernel timing of launch_block_subset: an event pair on the
+// launch's stream around every round, stop and post launch (diagnostic runs
+// only; a captured run passes none).
+struct LaunchTimer {
+  enum Kind { kRound = 0, kStop = 1, kPost = 2, kKinds = 3 };
+  std::vector<hipEvent_t> ev;  // pairs: start, end
+  std::vector<int> kind;       // one per pair
+  int begin(hipStream_t s, int k);  // records a start event, returns the pair index (-1 on failure)
+  void end(hipStream_t s, int pair);
+  ~LaunchTimer();
+};
+
 // Enqueues the whole per-source phase for one subset on `stream`.
 // round_tiles[r]: the tile units round r covers (the subset's blocks are
 // ordered by decreasing round bound, so they are a prefix).
@@ -97,7 +110,8 @@ struct BlockSubset {
 // post_slots: post kernel workgroups (resident ones; they loop over sources).
 int launch_block_subset(tgtp_ctx* ctx, hipStream_t stream, const BlockArgs& A,
                         const BlockSubset& S, const uint32_t* round_tiles,
-                        const uint32_t* round_hubs, uint32_t post_slots);
+                        const uint32_t* round_hubs, uint32_t post_slots,
+                        LaunchTimer* timer = nullptr);
 // Resident workgroups per CU of the post kernel launched with `stage`
 // floats of LDS vector buffer.
 int block_post_per_cu(uint32_t stage);

@@ -76,6 +76,8 @@ struct FastPlan {
   // source blocks (64 sources of one graph, largest graphs first) and the
   // subsets whose n x 64 states share the two state buffers
   BlockInfo* d_blocks = nullptr;
+  std::vector<BlockInfo> h_blocks;  // host copy (profiling)
+  std::vector<uint64_t> g_m;        // CSR entries per graph
   uint32_t n_blocks = 0;
   uint64_t n_sources = 0;
   std::vector<BlockSubset> subsets;
@@ -104,8 +106,10 @@ struct FastPlan {
   // tuned parameters
   uint32_t light = 32, small_bq = 8;
   // captured run
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // captured run: one graph, or four (pre, even subsets, odd subsets, post)
+  // for a plan on two streams
+  hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
   // batches: odd subsets on a second stream with their own state buffers,
   // so one subset's post kernel (latency-bound walks) overlaps the next
   // subset's rounds (bandwidth-bound)
@@ -119,8 +123,10 @@ struct FastPlan {
 
 void free_plan(FastPlan* P) {
   if (!P) return;
-  if (P->exec) (void)hipGraphExecDestroy(P->exec);
-  if (P->graph) (void)hipGraphDestroy(P->graph);
+  for (int k = 0; k < 4; ++k) {
+    if (P->exec[k]) (void)hipGraphExecDestroy(P->exec[k]);
+    if (P->graph[k]) (void)hipGraphDestroy(P->graph[k]);
+  }
   if (P->side) (void)hipStreamDestroy(P->side);
   if (P->fork) (void)hipEventDestroy(P->fork);
   if (P->join) (void)hipEventDestroy(P->join);
@@ -458,6 +464,9 @@ int build_plan(tgtp_ctx* ctx, tgtp_dgraph* dg, const std::vector<GraphSpec>& spe
     b0 = b;
   }
   const uint64_t cols = static_cast<uint64_t>(P->n_blocks) * kBlkCols;
+  P->h_blocks = blocks;
+  P->g_m.resize(G);
+  for (uint32_t k = 0; k < G; ++k) P->g_m[k] = specs[k].m;
   if ((rc = dev_upload(ctx, &P->d_blocks, blocks, "source blocks")) ||
       (rc = dev_alloc(ctx, &P->d_sa, (P->state_floats = max_floats), "diffusion state")) ||
       (rc = dev_alloc(ctx, &P->d_sb, max_floats, "diffusion state")) ||
@@ -532,10 +541,13 @@ TauFastArgs plan_tau_args(const FastPlan* P, tgtp_dgraph* dg) {
   return T;
 }
 
-// The run itself: device work only (capturable).  Events 5 and 6 (when
+// The run itself, device work only, in three parts so that a capture can
+// take each stream's share as a graph of its own: pre (clear, tau
+// projection) on the main stream, the subsets (`parity` -1: all on `st`;
+// 0 / 1: the even / odd ones, on the stream they run on), post
+// (symmetrization) on the main stream.  Events 5 and 6 (when
 // record_events) split tau / per-source / symmetrization.
-int enqueue_run(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_of,
-                bool record_events, const BlockArgs* override_args) {
+int enqueue_pre(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, bool record_events) {
   int rc;
   if ((rc = launch_fill_words(ctx, P->d_words, 16)) ||
       (rc = launch_fill_i32(ctx, P->d_taus, P->m, 0)) ||
@@ -543,18 +555,30 @@ int enqueue_run(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_
                              tgtp_mode_base(P->mode) == TGTP_MODE_FAST_BF16)))
     return rc;
   if (record_events) TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+  return TGTP_OK;
+}
+
+// Args of the main stream (second = false) or of the second stream, which
+// has its own state buffers and post-kernel work counter.
+BlockArgs stream_args(FastPlan* P, tgtp_dgraph* dg, const BlockArgs* override_args, bool second) {
   BlockArgs A = override_args ? *override_args : plan_args(P, dg);
   A.light = P->light;
   A.small_bq = P->small_bq;
-  const bool dual = P->dual && P->subsets.size() > 1;
-  BlockArgs A2 = A;
-  if (dual) {
-    A2.xa = P->d_sa2;
-    A2.xb = P->d_sb2;
-    A2.next_src = P->d_words + 4;
-    TGTP_HIP_CHECK(ctx, hipEventRecord(P->fork, ctx->stream));
-    TGTP_HIP_CHECK(ctx, hipStreamWaitEvent(P->side, P->fork, 0));
+  if (second) {
+    A.xa = P->d_sa2;
+    A.xb = P->d_sb2;
+    A.next_src = P->d_words + 4;
   }
+  return A;
+}
+
+// Subsets [0, count) whose index parity matches (`parity` -1: all; 0 / 1:
+// even / odd), on the stream of their parity when `dual` (odd on the second
+// stream), issued in subset order.
+int enqueue_subsets(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const BlockArgs* override_args,
+                    int parity, bool dual, LaunchTimer* timer) {
+  const BlockArgs A0 = stream_args(P, dg, override_args, false);
+  const BlockArgs A1 = stream_args(P, dg, override_args, true);
   // TGTP_PROGRESS=1: an uncaptured run waits for each subset and reports it
   // on stderr (runs of minutes on large graphs)
   bool progress = env_int_batch("TGTP_PROGRESS", 0) != 0;
@@ -565,14 +589,15 @@ int enqueue_run(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_
   }
   const double t_start = tgtp_now_ms();
   for (size_t i = 0; i < P->subsets.size(); ++i) {
-    const BlockSubset& S = P->subsets[i];
+    if (parity >= 0 && static_cast<int>(i & 1u) != parity) continue;
     const bool second = dual && (i & 1u);
     hipStream_t st = second ? P->side : ctx->stream;
-    if ((rc = launch_block_subset(ctx, st, second ? A2 : A, S,
-                                  P->round_tiles.data() + S.round_tiles0,
-                                  P->round_hubs.data() + S.round_tiles0,
-                                  std::max(1u, P->slots * S.post_per_cu / P->per_cu_staged))))
-      return rc;
+    const BlockSubset& S = P->subsets[i];
+    int rc = launch_block_subset(ctx, st, second ? A1 : A0, S,
+                                 P->round_tiles.data() + S.round_tiles0,
+                                 P->round_hubs.data() + S.round_tiles0,
+                                 std::max(1u, P->slots * S.post_per_cu / P->per_cu_staged), timer);
+    if (rc) return rc;
     if (progress) {
       TGTP_HIP_CHECK(ctx, hipStreamSynchronize(st));
       std::fprintf(stderr, "tgtp: source block subset %zu/%zu (%u rounds) done at %.1f s\n", i + 1,
@@ -580,12 +605,35 @@ int enqueue_run(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_
       std::fflush(stderr);
     }
   }
+  return TGTP_OK;
+}
+
+int enqueue_post(tgtp_ctx* ctx, FastPlan* P, const int32_t* src_of, bool record_events) {
+  if (record_events) TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
+  return launch_symmetrize_d(ctx, P->d_preds, src_of, P->m, P->d_out);
+}
+
+bool plan_dual(const FastPlan* P) { return P->dual && P->subsets.size() > 1; }
+
+// Uncaptured run: odd subsets on the second stream (its own buffer pair),
+// so one subset's post kernel (latency-bound walks) overlaps the next
+// subset's rounds (bandwidth-bound).
+int enqueue_run(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_of,
+                bool record_events, const BlockArgs* override_args,
+                LaunchTimer* timer = nullptr) {
+  int rc;
+  if ((rc = enqueue_pre(ctx, P, dg, record_events))) return rc;
+  const bool dual = plan_dual(P);
+  if (dual) {
+    TGTP_HIP_CHECK(ctx, hipEventRecord(P->fork, ctx->stream));
+    TGTP_HIP_CHECK(ctx, hipStreamWaitEvent(P->side, P->fork, 0));
+  }
+  if ((rc = enqueue_subsets(ctx, P, dg, override_args, -1, dual, timer))) return rc;
   if (dual) {
     TGTP_HIP_CHECK(ctx, hipEventRecord(P->join, P->side));
     TGTP_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, P->join, 0));
   }
-  if (record_events) TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
-  return launch_symmetrize_d(ctx, P->d_preds, src_of, P->m, P->d_out);
+  return enqueue_post(ctx, P, src_of, record_events);
 }
 
 // Second stream and state buffers for a plan with more than one subset
@@ -641,21 +689,63 @@ int autotune(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_of)
   return TGTP_OK;
 }
 
-int capture(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_of) {
-  TGTP_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  const int rc = enqueue_run(ctx, P, dg, src_of, false, nullptr);
+// Captures one part of the run on `st` into P->graph[k] / P->exec[k].
+template <typename F>
+int capture_part(tgtp_ctx* ctx, FastPlan* P, int k, hipStream_t st, F&& enqueue) {
+  TGTP_HIP_CHECK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue();
   hipGraph_t g = nullptr;
-  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  const hipError_t e = hipStreamEndCapture(st, &g);
   if (rc) {
     if (g) (void)hipGraphDestroy(g);
     return rc;
   }
   if (e != hipSuccess) return set_error(ctx, TGTP_E_HIP, "hipStreamEndCapture failed");
-  P->graph = g;
-  if (hipGraphInstantiate(&P->exec, g, nullptr, nullptr, 0) != hipSuccess) {
-    P->exec = nullptr;
+  P->graph[k] = g;
+  if (hipGraphInstantiate(&P->exec[k], g, nullptr, nullptr, 0) != hipSuccess) {
+    P->exec[k] = nullptr;
     return set_error(ctx, TGTP_E_HIP, "hipGraphInstantiate failed");
   }
+  return TGTP_OK;
+}
+
+// A single-stream plan is one graph.  A two-stream plan is captured as one
+// graph per stream share -- pre (main), even subsets (main), odd subsets
+// (second stream), post (main) -- replayed on their streams with the fork /
+// join events between them: one graph holding both streams' branches ran
+// its branches without their overlap (6% slower than the uncaptured run on
+// the 256-graph batch, while a one-stream graph matched its uncaptured run).
+int capture(tgtp_ctx* ctx, FastPlan* P, tgtp_dgraph* dg, const int32_t* src_of) {
+  if (!plan_dual(P))
+    return capture_part(ctx, P, 0, ctx->stream,
+                        [&] { return enqueue_run(ctx, P, dg, src_of, false, nullptr); });
+  int rc;
+  if ((rc = capture_part(ctx, P, 0, ctx->stream, [&] { return enqueue_pre(ctx, P, dg, false); })) ||
+      (rc = capture_part(ctx, P, 1, ctx->stream,
+                         [&] { return enqueue_subsets(ctx, P, dg, nullptr, 0, true, nullptr); })) ||
+      (rc = capture_part(ctx, P, 2, P->side,
+                         [&] { return enqueue_subsets(ctx, P, dg, nullptr, 1, true, nullptr); })) ||
+      (rc = capture_part(ctx, P, 3, ctx->stream,
+                         [&] { return enqueue_post(ctx, P, src_of, false); })))
+    return rc;
+  return TGTP_OK;
+}
+
+// Replays the captured run on the context's stream (and the second stream).
+int launch_captured(tgtp_ctx* ctx, FastPlan* P) {
+  hipStream_t s = ctx->stream;
+  if (!P->exec[1]) {
+    TGTP_HIP_CHECK(ctx, hipGraphLaunch(P->exec[0], s));
+    return TGTP_OK;
+  }
+  TGTP_HIP_CHECK(ctx, hipGraphLaunch(P->exec[0], s));
+  TGTP_HIP_CHECK(ctx, hipEventRecord(P->fork, s));
+  TGTP_HIP_CHECK(ctx, hipStreamWaitEvent(P->side, P->fork, 0));
+  TGTP_HIP_CHECK(ctx, hipGraphLaunch(P->exec[2], P->side));
+  TGTP_HIP_CHECK(ctx, hipEventRecord(P->join, P->side));
+  TGTP_HIP_CHECK(ctx, hipGraphLaunch(P->exec[1], s));
+  TGTP_HIP_CHECK(ctx, hipStreamWaitEvent(s, P->join, 0));
+  TGTP_HIP_CHECK(ctx, hipGraphLaunch(P->exec[3], s));
   return TGTP_OK;
 }
 
@@ -993,8 +1083,8 @@ extern "C" int tgtp_batch_run(tgtp_ctx* ctx, tgtp_batch* b, float* const* scores
   hipStream_t s = ctx->stream;
   auto* prep = static_cast<TgtpPrep*>(b->dg->prep);
   TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[4], s));
-  if (b->plan->exec) {
-    TGTP_HIP_CHECK(ctx, hipGraphLaunch(b->plan->exec, s));
+  if (b->plan->exec[0]) {
+    if ((rc = launch_captured(ctx, b->plan))) return rc;
   } else if ((rc = enqueue_run(ctx, b->plan, b->dg, prep->d_src_of, false, nullptr))) {
     return rc;
   }
@@ -1014,6 +1104,65 @@ extern "C" int tgtp_batch_run(tgtp_ctx* ctx, tgtp_batch* b, float* const* scores
     stats->work_items = b->plan->n_sources;
     stats->taus_ms = b->tune_ms;
   }
+  return TGTP_OK;
+}
+
+extern "C" int tgtp_batch_profile(tgtp_ctx* ctx, tgtp_batch* b, tgtp_profile* out) {
+  if (!ctx) return TGTP_E_ARG;
+  if (!b || !out) return set_error(ctx, TGTP_E_ARG, "null batch or output");
+  if (!b->plan) return set_error(ctx, TGTP_E_UNSUPPORTED, "profiles cover FAST batches only");
+  TGTP_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  FastPlan* P = b->plan;
+  auto* prep = static_cast<TgtpPrep*>(b->dg->prep);
+  hipStream_t s = ctx->stream;
+  LaunchTimer T;
+  TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[4], s));
+  int rc = enqueue_run(ctx, P, b->dg, prep->d_src_of, true, nullptr, &T);
+  if (rc) return rc;
+  TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[7], s));
+  TGTP_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  if (P->side) TGTP_HIP_CHECK(ctx, hipStreamSynchronize(P->side));
+  if ((rc = check_status(ctx, P))) return rc;
+  std::memset(out, 0, sizeof(*out));
+  float a = 0;
+  TGTP_HIP_CHECK(ctx, hipEventElapsedTime(&a, ctx->ev[4], ctx->ev[7]));
+  out->total_ms = a;
+  TGTP_HIP_CHECK(ctx, hipEventElapsedTime(&a, ctx->ev[4], ctx->ev[5]));
+  out->taus_ms = a;
+  double ms[LaunchTimer::kKinds] = {0, 0, 0};
+  uint64_t cnt[LaunchTimer::kKinds] = {0, 0, 0};
+  for (size_t i = 0; i < T.kind.size(); ++i) {
+    float e = 0;
+    TGTP_HIP_CHECK(ctx, hipEventElapsedTime(&e, T.ev[2 * i], T.ev[2 * i + 1]));
+    ms[T.kind[i]] += e;
+    ++cnt[T.kind[i]];
+  }
+  out->round_ms = ms[LaunchTimer::kRound];
+  out->round_launches = cnt[LaunchTimer::kRound];
+  out->stop_ms = ms[LaunchTimer::kStop];
+  out->stop_launches = cnt[LaunchTimer::kStop];
+  out->post_ms = ms[LaunchTimer::kPost];
+  out->post_launches = cnt[LaunchTimer::kPost];
+  // rounds each source (ell) and each block (last) took
+  const uint64_t cols = static_cast<uint64_t>(P->n_blocks) * kBlkCols;
+  std::vector<int32_t> ell(cols);
+  std::vector<uint32_t> last(P->n_blocks);
+  TGTP_HIP_CHECK(ctx, hipMemcpy(ell.data(), P->d_ell, cols * 4, hipMemcpyDeviceToHost));
+  TGTP_HIP_CHECK(ctx, hipMemcpy(last.data(), P->d_colu + 4 * cols, P->n_blocks * 4ull,
+                                hipMemcpyDeviceToHost));
+  uint64_t bytes = 0, src_rounds = 0;
+  for (uint32_t k = 0; k < P->n_blocks; ++k) {
+    const BlockInfo& B = P->h_blocks[k];
+    const uint64_t mg = P->g_m[B.gi], ng = B.n;
+    if (last[k] > 1) bytes += (last[k] - 1ull) * (4 * mg + 8 * ng);  // ids + row descriptors
+    for (uint32_t j = 0; j < B.cnt; ++j) {
+      const int32_t e = ell[static_cast<uint64_t>(k) * kBlkCols + j];
+      src_rounds += static_cast<uint64_t>(std::max(e, 0));
+      if (e > 1) bytes += (e - 1ull) * 4 * (mg + ng);  // its value per edge read, per row written
+    }
+  }
+  out->round_bytes = bytes;
+  out->source_rounds = src_rounds;
   return TGTP_OK;
 }
 

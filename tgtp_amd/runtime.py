@@ -77,6 +77,14 @@ class Batch:
                "tgtp_batch_run")
         return outs, st.as_dict()
 
+    def profile(self) -> dict:
+        """tgtp_batch_profile: per-kernel times of one uncaptured run and the
+        round kernel's algorithmic bytes."""
+        pr = L.ProfileStruct()
+        _check(self.ctx.ptr, L.lib().tgtp_batch_profile(self.ctx.ptr, self.ptr, C.byref(pr)),
+               "tgtp_batch_profile")
+        return pr.as_dict()
+
     def config(self) -> dict:
         v = [C.c_uint32() for _ in range(4)]
         rc = L.lib().tgtp_batch_config(self.ptr, *[C.byref(x) for x in v])
