@@ -154,6 +154,20 @@ int tgtp_mt_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,
                   uint32_t* counters_out, uint64_t* visit_count, uint64_t* checksum,
                   tgtp_stats* stats);
 
+/* tgtp_mt_walks without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, checks and results as tgtp_mt_walks; the device counters and
+ * sums live in a static 16 MiB scratch array that each GPU holds until the
+ * process ends, so graphs of at most 4,194,304 vertices are accepted
+ * (TGTP_E_UNSUPPORTED beyond).  Every call on one GPU uses that GPU's array
+ * (cleared at the start of each call): calls that overlap on one GPU, from
+ * any context or thread, race. */
+int tgtp_mt_walks_noalloc(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,
+                          uint32_t src_end, uint32_t depth, uint32_t n_walks, int32_t variant,
+                          uint32_t* counters_out, uint64_t* visit_count, uint64_t* checksum,
+                          tgtp_stats* stats);
+
 /* ---- tau bounds (calculate_taus, aesc.h:164-216) ---------------------------
  * eps_half is float(epsilon/2) as aesc.h:122 passes it.  taus_out [m]. */
 int tgtp_taus(tgtp_ctx* ctx, const tgtp_graph* g, const tgtp_eigens* e,

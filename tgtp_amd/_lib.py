@@ -83,7 +83,8 @@ class ProfileStruct(C.Structure):
 # every symbol the two public headers declare
 EXPORTED = [
     "tgtp_ctx_create", "tgtp_ctx_destroy", "tgtp_ctx_last_error", "tgtp_strerror",
-    "tgtp_graph_upload", "tgtp_graph_free", "tgtp_saba_walks", "tgtp_mt_walks", "tgtp_taus",
+    "tgtp_graph_upload", "tgtp_graph_free", "tgtp_saba_walks", "tgtp_mt_walks",
+    "tgtp_mt_walks_noalloc", "tgtp_taus",
     "tgtp_score", "tgtp_score_dev", "tgtp_score_trace", "tgtp_score_sources",
     "tgtp_batch_create", "tgtp_batch_run", "tgtp_batch_destroy", "tgtp_batch_config",
     "tgtp_batch_profile",
@@ -132,6 +133,8 @@ def lib() -> C.CDLL:
                                   P(C.c_uint32), P(C.c_uint64), P(C.c_uint64), P(StatsStruct)]
     L.tgtp_mt_walks.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32,
                                 P(C.c_uint32), P(C.c_uint64), P(C.c_uint64), P(StatsStruct)]
+    L.tgtp_mt_walks_noalloc.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32,
+                                        P(C.c_uint32), P(C.c_uint64), P(C.c_uint64), P(StatsStruct)]
     L.tgtp_taus.argtypes = [vp, P(GraphStruct), P(EigensStruct), C.c_float, C.c_int32,
                             P(C.c_int32), P(StatsStruct)]
     L.tgtp_score.argtypes = [vp, P(GraphStruct), P(EigensStruct), P(ParamsStruct),

@@ -2,12 +2,38 @@
 // argument validation and dispatch to the kernels.
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 
 #include "common.hpp"
 
+// Scratch of tgtp_mt_walks_noalloc: the walk counters of at most
+// kNoAllocMaxN vertices, then the two 64-bit sums.  Every GPU the library
+// runs on holds its own copy (16 MiB) until the process ends.  In host code
+// `scratch` is only a stand-in for the device variable: its address on each
+// device comes from hipGetSymbolAddress, once per device, in
+// tgtp_ctx_create.
+constexpr uint64_t kNoAllocMaxN = 1ull << 22;
+static __device__ __align__(256) uint32_t scratch[kNoAllocMaxN + 64];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// This device's address of `scratch` (the device must be current); null if
+// the runtime cannot resolve it.
+void* device_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch_ptr[device] = p;
+  }
+  return g_scratch_ptr[device];
+}
 
 __global__ void build_rowd_kernel(const uint32_t* __restrict__ idx, uint64_t n,
                                   uint2* __restrict__ rowd, float* __restrict__ degf) {
@@ -98,6 +124,7 @@ extern "C" int tgtp_ctx_create(int device, tgtp_ctx** out) {
     return TGTP_E_HIP;
   }
   ctx->num_cus = prop.multiProcessorCount;
+  ctx->scratch = device_scratch(device);  // checked by the entry point that uses it
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return TGTP_E_HIP;
@@ -174,11 +201,10 @@ extern "C" void tgtp_graph_free(tgtp_ctx* ctx, tgtp_dgraph* dg) {
   delete dg;
 }
 
-extern "C" int tgtp_mt_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,
-                             uint32_t src_end, uint32_t depth, uint32_t n_walks, int32_t variant,
-                             uint32_t* counters_out, uint64_t* visit_count, uint64_t* checksum,
-                             tgtp_stats* stats) {
-  if (!ctx) return TGTP_E_ARG;
+// Shared body of tgtp_mt_walks and tgtp_mt_walks_noalloc: validation, then
+// the walks with the given device counters [n] and sums [2].
+static int mt_walks_check(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,
+                          uint32_t src_end, uint32_t n_walks, int32_t variant) {
   if (!dg) return set_error(ctx, TGTP_E_ARG, "null device graph");
   if (variant != TGTP_WALK_MT_REGULAR && variant != TGTP_WALK_MT_SIMD)
     return set_error(ctx, TGTP_E_ARG, "unknown walk variant");
@@ -186,12 +212,13 @@ extern "C" int tgtp_mt_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_
     return set_error(ctx, TGTP_E_ARG, "n_walks must be a multiple of 8");
   if (src_end > dg->n || src_begin > src_end)
     return set_error(ctx, TGTP_E_ARG, "source range outside [0, n]");
-  const double t0 = now_ms();
-  TGTP_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-  uint32_t* d_counters =
-      static_cast<uint32_t*>(ctx_buffer(ctx, kBufCounters, dg->n * sizeof(uint32_t)));
-  auto* d_sums = static_cast<unsigned long long*>(ctx_buffer(ctx, kBufSums, 64));
-  if (!d_counters || !d_sums) return set_error(ctx, TGTP_E_NOMEM, "counter buffers");
+  return TGTP_OK;
+}
+
+static int mt_walks_run(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin, uint32_t src_end,
+                        uint32_t depth, uint32_t n_walks, int32_t variant, uint32_t* counters_out,
+                        uint64_t* visit_count, uint64_t* checksum, tgtp_stats* stats,
+                        uint32_t* d_counters, unsigned long long* d_sums, double t0) {
   float walk_ms = 0;
   int rc = launch_mt_walks(ctx, dg, src_begin, src_end, depth, n_walks, variant, d_counters,
                            d_sums, &walk_ms);
@@ -216,6 +243,49 @@ extern "C" int tgtp_mt_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_
     stats->e2e_ms = now_ms() - t0;
   }
   return TGTP_OK;
+}
+
+extern "C" int tgtp_mt_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,
+                             uint32_t src_end, uint32_t depth, uint32_t n_walks, int32_t variant,
+                             uint32_t* counters_out, uint64_t* visit_count, uint64_t* checksum,
+                             tgtp_stats* stats) {
+  if (!ctx) return TGTP_E_ARG;
+  int rc = mt_walks_check(ctx, dg, src_begin, src_end, n_walks, variant);
+  if (rc) return rc;
+  const double t0 = now_ms();
+  TGTP_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  uint32_t* d_counters =
+      static_cast<uint32_t*>(ctx_buffer(ctx, kBufCounters, dg->n * sizeof(uint32_t)));
+  auto* d_sums = static_cast<unsigned long long*>(ctx_buffer(ctx, kBufSums, 64));
+  if (!d_counters || !d_sums) return set_error(ctx, TGTP_E_NOMEM, "counter buffers");
+  return mt_walks_run(ctx, dg, src_begin, src_end, depth, n_walks, variant, counters_out,
+                      visit_count, checksum, stats, d_counters, d_sums, t0);
+}
+
+extern "C" int tgtp_mt_walks_noalloc(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,
+                                     uint32_t src_end, uint32_t depth, uint32_t n_walks,
+                                     int32_t variant, uint32_t* counters_out,
+                                     uint64_t* visit_count, uint64_t* checksum,
+                                     tgtp_stats* stats) {
+  if (!ctx) return TGTP_E_ARG;
+  int rc = mt_walks_check(ctx, dg, src_begin, src_end, n_walks, variant);
+  if (rc) return rc;
+  if (dg->n > kNoAllocMaxN)
+    return set_error(ctx, TGTP_E_UNSUPPORTED,
+                     "tgtp_mt_walks_noalloc: the graph has " + std::to_string(dg->n) +
+                         " vertices; its scratch holds counters for at most " +
+                         std::to_string(kNoAllocMaxN) + " (use tgtp_mt_walks)");
+  if (!ctx->scratch)
+    return set_error(ctx, TGTP_E_HIP, "tgtp_mt_walks_noalloc: device scratch address unavailable");
+  const double t0 = now_ms();
+  TGTP_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  // counters from the start of this device's scratch, the sums after the
+  // largest counter array (an offset of 16 MiB: 8-byte aligned); both are
+  // cleared by the walk's first kernel
+  auto* d_counters = static_cast<uint32_t*>(ctx->scratch);
+  auto* d_sums = reinterpret_cast<unsigned long long*>(d_counters + kNoAllocMaxN);
+  return mt_walks_run(ctx, dg, src_begin, src_end, depth, n_walks, variant, counters_out,
+                      visit_count, checksum, stats, d_counters, d_sums, t0);
 }
 
 extern "C" int tgtp_saba_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,

@@ -76,6 +76,7 @@ struct tgtp_ctx {
   hipEvent_t ev[8] = {};
   std::string err;
   DevBuf bufs[16];
+  void* scratch = nullptr;  // this device's copy of capi.hip's static scratch (no-allocation entry)
 };
 
 // Scratch slots in tgtp_ctx::bufs.

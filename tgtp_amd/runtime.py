@@ -151,17 +151,20 @@ class Context:
         return cnt, int(vc.value), int(cs.value), st.as_dict()
 
     def mt_walks(self, dg: DeviceGraph, depth: int, n_walks: int, variant: int = L.WALK_MT_REGULAR,
-                 src_begin: int = 0, src_end: int | None = None, want_counters: bool = True):
-        """MT19937 comparator walks (tgtp_mt_walks) -> (counters uint32[n] or
-        None, visit_count, checksum, stats dict)."""
+                 src_begin: int = 0, src_end: int | None = None, want_counters: bool = True,
+                 noalloc: bool = False):
+        """MT19937 comparator walks (tgtp_mt_walks, or tgtp_mt_walks_noalloc
+        with noalloc=True) -> (counters uint32[n] or None, visit_count,
+        checksum, stats dict)."""
         src_end = dg.n if src_end is None else src_end
         cnt = np.zeros(dg.n, dtype=np.uint32) if want_counters else None
         vc, cs = C.c_uint64(), C.c_uint64()
         st = L.StatsStruct()
-        rc = L.lib().tgtp_mt_walks(self.ptr, dg.ptr, src_begin, src_end, depth, n_walks, variant,
-                                   L.ptr(cnt, C.c_uint32) if cnt is not None else None,
-                                   C.byref(vc), C.byref(cs), C.byref(st))
-        _check(self.ptr, rc, "tgtp_mt_walks")
+        fn = L.lib().tgtp_mt_walks_noalloc if noalloc else L.lib().tgtp_mt_walks
+        rc = fn(self.ptr, dg.ptr, src_begin, src_end, depth, n_walks, variant,
+                L.ptr(cnt, C.c_uint32) if cnt is not None else None,
+                C.byref(vc), C.byref(cs), C.byref(st))
+        _check(self.ptr, rc, "tgtp_mt_walks_noalloc" if noalloc else "tgtp_mt_walks")
         return cnt, int(vc.value), int(cs.value), st.as_dict()
 
     @staticmethod
