@@ -127,7 +127,8 @@ void tgtp_graph_free(tgtp_ctx* ctx, tgtp_dgraph* dg);
  * reference keeps per-thread float counters, rw_experiment_saba.cpp:192).
  * counters_out [n] may be NULL; visit_count = sum(count), checksum =
  * sum((v+1)*count[v]) in uint64 (rw_experiment_saba.cpp:203-206).
- * n_walks must be a positive multiple of 8 (rw_experiment_saba.cpp:160-163).
+ * n_walks must be a multiple of 8 (rw_experiment_saba.cpp:160-163); 0 walks
+ * nothing and returns zero counters and sums, as the reference does (:186).
  * stats->kernel_ms covers counter clearing + walks + reduction, the window
  * of t_rw_s (rw_experiment_saba.cpp:184-211) minus host work. */
 int tgtp_saba_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,

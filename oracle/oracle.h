@@ -73,7 +73,7 @@ int orc_saba_walks(uint64_t n, const uint32_t* idx, const uint32_t* adj,
 /* The same walks with the 8 walkers of a bouquet in one AVX2 register
  * (orc_walk_avx2.cpp): equal counters, the host CPU baseline of the walk
  * engine.  Returns -2 where the CPU lacks AVX2, -1 for n_walks not a
- * positive multiple of 8 or a graph beyond 2^31 vertices / entries. */
+ * multiple of 8 or a graph beyond 2^31 vertices / entries. */
 int orc_saba_walks_avx2(uint64_t n, const uint32_t* idx, const uint32_t* adj,
                         uint32_t src_begin, uint32_t src_end, uint32_t depth,
                         uint64_t n_walks, uint32_t* counters, int threads, int chunk);

@@ -125,7 +125,7 @@ extern "C" int orc_saba_walks_avx2(uint64_t n, const uint32_t* idx, const uint32
                                    int chunk) {
   if (!__builtin_cpu_supports("avx2")) return -2;
   // gathers use signed 32-bit indices
-  if (n_walks == 0 || n_walks % kLanes != 0 || n_walks > 0xFFFFFFFFull || n >= (1ull << 31) ||
+  if (n_walks % kLanes != 0 || n_walks > 0xFFFFFFFFull || n >= (1ull << 31) ||
       idx[n] >= (1u << 31))
     return -1;
   if (src_end > n) src_end = static_cast<uint32_t>(n);
@@ -137,7 +137,7 @@ extern "C" int orc_saba_walks_avx2(uint64_t n, const uint32_t* idx, const uint32
 #pragma omp for schedule(dynamic, chunk)
     for (int64_t s = src_begin; s < static_cast<int64_t>(src_end); ++s) {
       const uint32_t src = static_cast<uint32_t>(s);
-      if (idx[src + 1] == idx[src]) continue;
+      if (idx[src + 1] == idx[src] || n_walks == 0) continue;
       walk_source_avx2(idx, adj, src, depth, static_cast<uint32_t>(n_walks), local.data());
     }
 #pragma omp critical

@@ -2,10 +2,12 @@
 tree is present; the tests themselves only read the .npz files).
 
 Sources of truth:
-  * saba_kats.json: known answers of the reference's own saba_randomwalk
-    (src/rw_experiment_saba.cpp:65-145) recorded from a build of its own source:
-    C9 (40 hops, 64 walks) and K5 (20 hops, 64
-    walks), both forcing the whole-bouquet SplitMix32 fallback.
+  * saba_kats.json (not written here): known answers of the reference's own
+    saba_randomwalk (src/rw_experiment_saba.cpp:65-145), recorded from a
+    build of its own source with a lane-wise % in place of Intel SVML's
+    _mm256_rem_epu32 (which only icpx provides): C9 (40 hops, 64 walks) and
+    K5 (20 hops, 64 walks), both forcing the whole-bouquet SplitMix32
+    fallback.
   * ref_rwmt_<case>.npz: per-vertex visit counts of the reference's own
     scalar_randomwalk (src/rw_experiment.cpp:34-47, per-source std::mt19937
     reseeded 42 + source) through oracle/_ref/ref_mt_walk, for several

@@ -2,6 +2,7 @@
 reference CLIs (main.cpp:9-32, benchmark.cpp:10-28,
 rw_experiment_saba.cpp:150-165); GPU-marked tests check their outputs
 against the oracle."""
+import os
 import subprocess
 from pathlib import Path
 
@@ -15,8 +16,8 @@ from tgtp_amd import graph as G
 BIN = Path(__file__).resolve().parents[1] / "tgtp_amd" / "bin"
 
 
-def run(args, cwd=None):
-    return subprocess.run([str(a) for a in args], capture_output=True, text=True, cwd=cwd)
+def run(args, cwd=None, env=None):
+    return subprocess.run([str(a) for a in args], capture_output=True, text=True, cwd=cwd, env=env)
 
 
 @pytest.mark.parametrize("argv,msg", [
@@ -181,6 +182,16 @@ def test_rw_cli_tsv_row(tmp_path):
     assert cols[12] == f"{vc:g}" and cols[13] == f"{cs:g}"
     res = (tmp_path / "results" / "rw" / "saba_cl1000_d5_w64.txt").read_text().splitlines()
     kv = dict(line.split(" ", 1) for line in res)
+    # the reference's keys, in its order (rw_experiment_saba.cpp:224-238)
+    assert list(kv) == ["variant", "dataset", "depth", "n_walks", "threads", "n_vertices",
+                        "n_edges", "t_load_s", "t_rw_s", "total_s", "total_walks",
+                        "walks_per_sec", "visit_count", "checksum"]
+    # TGTP_RW_EXACT=1 adds the exact integer sums
+    r = run([BIN / "rw_experiment_saba", tmp_path / "data", "cl1000", "5", "64", "4"], cwd=tmp_path,
+            env={**os.environ, "TGTP_RW_EXACT": "1"})
+    assert r.returncode == 0, r.stderr
+    res = (tmp_path / "results" / "rw" / "saba_cl1000_d5_w64.txt").read_text().splitlines()
+    kv = dict(line.split(" ", 1) for line in res)
     assert int(kv["visit_count_exact"]) == int(cnt.sum())
 
 
@@ -199,7 +210,8 @@ def test_mt_rw_cli_tsv_row(tmp_path, binary, variant, label):
     d = tmp_path / "data" / "cl1000"
     d.mkdir(parents=True)
     g.write_txt(d / "graph.txt")
-    r = run([BIN / binary, tmp_path / "data", "cl1000", "5", "64", "2"], cwd=tmp_path)
+    r = run([BIN / binary, tmp_path / "data", "cl1000", "5", "64", "2"], cwd=tmp_path,
+            env={**os.environ, "TGTP_RW_EXACT": "1"})
     assert r.returncode == 0, r.stderr
     cols = r.stdout.rstrip("\n").split("\t")
     assert len(cols) == 14 and cols[0] == label and cols[1] == "cl1000"

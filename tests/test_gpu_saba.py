@@ -130,6 +130,20 @@ def test_invalid_arguments_rejected(gpu_ctx):
     dg.free()
 
 
+def test_zero_walks_give_zero_counters(gpu_ctx):
+    """n_walks = 0 walks nothing and returns zero counters and sums, as the
+    reference's driver does (rw_experiment_saba.cpp:160, 186), also right
+    after a call that left counts in the context's buffers."""
+    g = G.cycle(9)
+    dg = gpu_ctx.upload(g)
+    try:
+        gpu_ctx.saba_walks(dg, 5, 8)
+        cnt, vc, cs, _ = gpu_ctx.saba_walks(dg, 5, 0)
+    finally:
+        dg.free()
+    assert vc == 0 and cs == 0 and not cnt.any()
+
+
 @pytest.fixture(scope="module")
 def cl1m():
     return W.chung_lu_1m()

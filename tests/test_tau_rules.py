@@ -46,7 +46,13 @@ def test_truncation_bound_holds_on_full_spectrum(seed):
     missed part after tau steps is below truncation_bound for odd tau."""
     g = G.chung_lu(180, 2.4, 6.0, seed=seed)
     omega = 24
-    lam_all, X_all = E.compute_eigens(g, g.n)  # every pair, sorted by |lambda|
+    # every pair (compute_eigens takes omega < n, as eigenprep.h does), in
+    # compute_eigens' layout: descending |lambda|, x = v sqrt(2 m_u / d)
+    P, rs, m_u = E.normalized_adjacency(g)
+    w, V = np.linalg.eigh(P.toarray())
+    order = np.argsort(-np.abs(w), kind="stable")
+    lam_all = w[order]
+    X_all = V[:, order] * (np.sqrt(2.0 * m_u) / np.sqrt(rs))[:, None]
     R = exact_sc.exact_scores(g, "cpu")
     rows, und = TR._edges(g)
     U, V = rows[und], g.adj[und].astype(np.int64)
@@ -92,3 +98,17 @@ def test_reference_algorithm_error_on_bridge_edges(batch_small):
         tb = TR.taus(g, lam, X, eps / 2, bounded=True)
         rows = oracle.tgtp_rows(g.idx, g.adj, tb, eps, 10, 0, g.n)
         assert np.abs(rows + rows[rev] - R).max() < 0.055
+
+
+def test_omega_range_as_eigenprep():
+    """omega must satisfy 2 <= omega < n (eigenprep.h:99-103), in both the
+    host and the block Krylov solvers."""
+    g = G.cycle(9)
+    for bad in (1, 9, 10):
+        with pytest.raises(ValueError, match="2 <= omega < n"):
+            E.compute_eigens(g, bad)
+    P, _, _ = E.normalized_adjacency(g)
+    with pytest.raises(ValueError, match="2 <= omega < n"):
+        E.block_krylov_eigs(E._torch_csr(P, "cpu"), 9)
+    w, _ = E.compute_eigens(g, 8)
+    assert w.shape == (8,)

@@ -294,7 +294,9 @@ extern "C" int tgtp_saba_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t sr
                                uint64_t* checksum, tgtp_stats* stats) {
   if (!ctx) return TGTP_E_ARG;
   if (!dg) return set_error(ctx, TGTP_E_ARG, "null device graph");
-  if (n_walks == 0 || n_walks % 8 != 0)
+  // n_walks == 0 walks nothing: counters and sums are zero, as the
+  // reference's driver gives (rw_experiment_saba.cpp:160, 186)
+  if (n_walks % 8 != 0)
     return set_error(ctx, TGTP_E_ARG,
                      "n_walks must be a multiple of 8 within the 32-bit grid domain");
   if (src_end > dg->n || src_begin > src_end)

@@ -10,6 +10,7 @@
 // t_load_s covers parsing and the upload to HBM; t_rw_s covers counter
 // clearing, the walks and the counter reduction (the reference's window).
 #include <chrono>
+#include <cstdlib>
 #include <cstdint>
 #include <filesystem>
 #include <fstream>
@@ -115,9 +116,14 @@ int main(int argc, char** argv) {
           << "total_walks " << total_walks << "\n"
           << "walks_per_sec " << walks_per_sec << "\n"
           << "visit_count " << visit_count << "\n"
-          << "checksum " << checksum << "\n"
-          << "visit_count_exact " << vc << "\n"
-          << "checksum_exact " << cs << "\n";
+          << "checksum " << checksum << "\n";
+      // TGTP_RW_EXACT=1 appends the exact integer sums (the double lines
+      // above round beyond 2^53); off by default, so the file has the
+      // reference's keys (rw_experiment_saba.cpp:224-238)
+      const char* ex = std::getenv("TGTP_RW_EXACT");
+      if (ex && std::string(ex) == "1")
+        out << "visit_count_exact " << vc << "\n"
+            << "checksum_exact " << cs << "\n";
     }
     std::cout << kVariant << "\t" << dataset << "\t" << depth << "\t" << n_walks << "\t" << threads
               << "\t" << n << "\t" << m << "\t" << t_load << "\t" << t_rw << "\t" << total << "\t"

@@ -50,6 +50,10 @@ struct tgtp_dgraph {
   // first walk call when the three fields fit in 64 bits
   uint64_t* walk_adj = nullptr;  // [m]
   uint32_t id_bits = 0, start_bits = 0;
+  // degree share of each cold vertex range of the bucketed walk counting,
+  // computed once per (hot, bshift) from the row degrees
+  std::vector<double> walk_mass;
+  uint32_t walk_mass_hot = 0, walk_mass_bshift = 0;
   // tgtp host-side state (host CSR copy, symmetrization map), built lazily
   void* prep = nullptr;
   void (*prep_free)(void*) = nullptr;

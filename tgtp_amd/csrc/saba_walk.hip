@@ -402,8 +402,12 @@ const void* walk_kernel_ptr(bool packed, bool bucket) {
 // Expected share of visits per cold bucket, from the degree mass of its
 // vertex range (a walk's visits follow the degrees once it has mixed); only
 // sizes the record regions, a region that fills up falls back to atomics.
+// Depends on the graph alone: kept on the dgraph for the next call.
 std::vector<double> bucket_mass(tgtp_ctx* ctx, tgtp_dgraph* dg, uint32_t hot, uint32_t bshift,
                                 uint32_t nb) {
+  if (!dg->walk_mass.empty() && dg->walk_mass.size() == nb && dg->walk_mass_hot == hot &&
+      dg->walk_mass_bshift == bshift)
+    return dg->walk_mass;
   std::vector<double> mass(nb, 0.0);
   std::vector<uint2> rd(dg->n);
   if (hipMemcpy(rd.data(), dg->rowd, dg->n * sizeof(uint2), hipMemcpyDeviceToHost) != hipSuccess)
@@ -415,6 +419,9 @@ std::vector<double> bucket_mass(tgtp_ctx* ctx, tgtp_dgraph* dg, uint32_t hot, ui
   }
   for (auto& x : mass) x = tot > 0 ? x / tot : 0.0;
   (void)ctx;
+  dg->walk_mass = mass;
+  dg->walk_mass_hot = hot;
+  dg->walk_mass_bshift = bshift;
   return mass;
 }
 
@@ -516,12 +523,14 @@ int launch_saba_walks(tgtp_ctx* ctx, const tgtp_dgraph* cdg, uint32_t src_begin,
     HistParams Q{};
     if (bucket) {
       waves = resident * kWalkWaves;
-      size_t free_b = 0, total_b = 0;
-      TGTP_HIP_CHECK(ctx, hipMemGetInfo(&free_b, &total_b));
       const uint64_t have = ctx->bufs[kBufWalkRec].bytes;
-      const uint64_t budget = std::min<uint64_t>(
-          static_cast<uint64_t>(env_int("TGTP_WALK_REC_MIB", 16384)) << 20,
-          std::max<uint64_t>(have, (free_b + have) / 4));
+      const uint64_t cap = static_cast<uint64_t>(env_int("TGTP_WALK_REC_MIB", 16384)) << 20;
+      uint64_t budget = cap;
+      if (have < cap) {  // the record regions have not reached their cap yet
+        size_t free_b = 0, total_b = 0;
+        TGTP_HIP_CHECK(ctx, hipMemGetInfo(&free_b, &total_b));
+        budget = std::min<uint64_t>(cap, std::max<uint64_t>(have, (free_b + have) / 4));
+      }
       double cold_share = 0.0;
       for (double x : mass) cold_share += x;
       const double per_src = static_cast<double>(n_walks) * depth;

@@ -46,8 +46,8 @@ def compute_eigens(g: Graph, omega: int, dense_limit: int = 2500, tol: float = 1
     """(values f64[omega], vectors f64[n, omega]) sorted by descending |lambda|."""
     P, rs, m_unique = normalized_adjacency(g)
     n = g.n
-    if not (2 <= omega <= n):
-        raise ValueError("omega must satisfy 2 <= omega <= n")
+    if not (2 <= omega < n):  # eigenprep.h:99-103
+        raise ValueError(f"omega must satisfy 2 <= omega < n; got {omega} for n={n}")
     if n <= dense_limit or omega >= n - 1:
         w, V = np.linalg.eigh(P.toarray())
     else:
@@ -120,8 +120,8 @@ def block_krylov_eigs(P, omega: int, tol: float = 1e-9, block: int = 64, basis: 
 
     n = P.shape[0]
     dev = P.device
-    if not (2 <= omega <= n):
-        raise ValueError("omega must satisfy 2 <= omega <= n")
+    if not (2 <= omega < n):  # eigenprep.h:99-103
+        raise ValueError(f"omega must satisfy 2 <= omega < n; got {omega} for n={n}")
     keep = keep or omega + max(omega, 64)
     basis = min(basis, n)
     block = min(block, max(1, basis - omega))
@@ -339,8 +339,8 @@ def compute_eigens_device(g: Graph, omega: int, device: str = "cuda:0", tol: flo
 
     P, rs, m_unique = normalized_adjacency(g)
     n = g.n
-    if not (2 <= omega <= n):
-        raise ValueError("omega must satisfy 2 <= omega <= n")
+    if not (2 <= omega < n):  # eigenprep.h:99-103
+        raise ValueError(f"omega must satisfy 2 <= omega < n; got {omega} for n={n}")
     if n <= dense_limit or omega >= n // 2:
         w, V = torch.linalg.eigh(torch.from_numpy(P.toarray()).to(device))
         order = torch.argsort(-w.abs(), stable=True)[:omega]
