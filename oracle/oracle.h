@@ -28,8 +28,11 @@
  *       known answer (orc_mt19937_nth(5489, 10000) == 4123659995).
  *   orc_mt_walks variant 1 (simd): the reference's simd_randomwalk needs
  *       Intel SVML (_mm256_rem_epu32), so it is not built; at depth 1 both
- *       variants draw the same outputs and must agree (tested).  The hop
- *       order of the draws at depth > 1 is PARITY UNPINNED.
+ *       variants draw the same outputs and must agree (tested); at depth > 1
+ *       the draw order (8 draws per hop in lane order,
+ *       rw_experiment_simd.cpp:55-57) is pinned by brute-force restatements
+ *       on std::mt19937 streams restated in numpy, on graphs where the
+ *       regular variant's order gives other counts.
  *   orc_saba_walks / orc_mt_walks on a CSR without both directions of every
  *       edge (the reference divides by zero there): a walker at a vertex
  *       without neighbours stops; checked against brute-force Python
@@ -39,11 +42,11 @@
  *       path pinned above.  Phase 1 of the 2-way walk is checked against the
  *       pinned 1-way walk (same grid, hash and multiply-high step).
  *       The fallback phase of the 2-way walk (two successive SplitMix32 draws
- *       from one state, aesc.h:339-340): that both walkers share one state is
- *       checked by an identical-endpoints property (they part only after the
- *       fallback); which walker takes the first draw is PARITY UNPINNED: the
- *       only reference code that runs it needs Intel SVML, which this image
- *       lacks.
+ *       from one state, aesc.h:339-340, the first for the walker from s):
+ *       pinned by a brute-force restatement of aesc.h:241-391 on graphs of
+ *       power-of-two degrees (a 16-leaf star, the 4-cube), where every
+ *       bouquet falls back and the restatement with the draws swapped gives
+ *       a different sum (the oracle with its draws swapped fails that test).
  */
 #ifndef TGTP_ORACLE_H
 #define TGTP_ORACLE_H

@@ -274,3 +274,118 @@ def test_avx2_walks_equal_scalar(case):
             assert np.array_equal(got, ref), f"{case} threads={threads} chunk={chunk}"
     except RuntimeError:
         pytest.skip("no AVX2 on this host")
+
+
+def walk2_brute_force(idx, adj, s, v, nr, length, pvec, swap=False):
+    """randomwalk_2way_avx2 (aesc.h:241-391) restated lane by lane from the
+    reference's text: both walkers' grid secrets start equal (:259-266); the
+    whole bouquet leaves the hashed phase when any lane of either secret is
+    zero (:272-277); the fallback draws two SplitMix32 outputs per lane from
+    one state, the first for the walker from s, the second for the walker
+    from v (:339-340; swap=True gives the other order); each lane adds
+    pvec[cur] - pvec[cur2] in float32 and the lanes are summed 0..7
+    (:320-330, 386-390).  Returns (x, whether some bouquet fell back)."""
+    deg = lambda u: int(idx[u + 1] - idx[u])  # noqa: E731
+    f32 = np.float32
+    step = (0xFFFFFFFF // nr) & 0xFFFFFFFF
+    x = [f32(0.0)] * 8
+    fell = False
+    for i in range(0, nr, 8):
+        c1, c2 = [s] * 8, [v] * 8
+        s1 = [((i + b + 1) * step) & 0xFFFFFFFF for b in range(8)]
+        s2 = list(s1)
+        st = list(s1)
+        hop = 0
+        while hop < length and not any(a == 0 for a in s1 + s2):
+            for b in range(8):
+                d1, d2 = deg(c1[b]), deg(c2[b])
+                k1 = ((s1[b] ^ _py_fmix32(c1[b])) * d1) >> 32
+                k2 = ((s2[b] ^ _py_fmix32(c2[b])) * d2) >> 32
+                c1[b] = int(adj[idx[c1[b]] + k1])
+                c2[b] = int(adj[idx[c2[b]] + k2])
+                s1[b] = (s1[b] * d1) & 0xFFFFFFFF
+                s2[b] = (s2[b] * d2) & 0xFFFFFFFF
+                x[b] = f32(x[b] + f32(pvec[c1[b]] - pvec[c2[b]]))
+            hop += 1
+        fell |= hop < length
+        for _ in range(hop, length):
+            for b in range(8):
+                r1, st[b] = _py_splitmix32(st[b])
+                r2, st[b] = _py_splitmix32(st[b])
+                if swap:
+                    r1, r2 = r2, r1
+                c1[b] = int(adj[idx[c1[b]] + r1 % deg(c1[b])])
+                c2[b] = int(adj[idx[c2[b]] + r2 % deg(c2[b])])
+                x[b] = f32(x[b] + f32(pvec[c1[b]] - pvec[c2[b]]))
+    total = f32(0.0)
+    for b in range(8):
+        total = f32(total + x[b])
+    return total, fell
+
+
+def _star(leaves):
+    edges = [(0, k) for k in range(1, leaves + 1)]
+    return G.Graph.from_edges(leaves + 1, edges)
+
+
+def _hypercube(dim):
+    n = 1 << dim
+    return G.Graph.from_edges(n, [(u, u ^ (1 << b)) for u in range(n) for b in range(dim)
+                                  if u < u ^ (1 << b)])
+
+
+@pytest.mark.parametrize("graph", ["star16", "q4"])
+def test_walk2_fallback_draw_order_pinned(graph):
+    """Graphs of power-of-two degrees drive every secret to zero (the secret
+    is multiplied by the degree each hop), so every bouquet reaches the
+    fallback.  On the 16-leaf star one walker is always at the hub and the
+    other at a leaf (a forced step), so the hub walker's draw -- the first or
+    the second of the hop -- decides the leaf it visits; on the 4-cube both
+    draw.  The oracle equals the restatement of the reference's order, and
+    the restatement with the two draws swapped differs from it, so a swap in
+    the oracle would fail here."""
+    g = _star(16) if graph == "star16" else _hypercube(4)
+    rng = np.random.default_rng(17)
+    pvec = rng.random(g.n).astype(np.float32)
+    pairs = [(0, 3), (5, 0)] if graph == "star16" else [(0, 1), (6, 14)]
+    swapped_differs = 0
+    for s, v in pairs:
+        for nr in (8, 64):
+            exp, fell = walk2_brute_force(g.idx, g.adj, s, v, nr, 40, pvec)
+            got = oracle.walk2_saba(g.idx, g.adj, s, v, nr, 40, pvec)
+            assert fell
+            assert np.float32(got).view(np.uint32) == exp.view(np.uint32), (s, v, nr)
+            alt, _ = walk2_brute_force(g.idx, g.adj, s, v, nr, 40, pvec, swap=True)
+            swapped_differs += int(alt != exp)
+    assert swapped_differs == 2 * len(pairs)
+
+
+def test_simd_comparator_hop_order_discriminates():
+    """The SIMD comparator draws 8 outputs per hop, one per lane in lane
+    order (rw_experiment_simd.cpp:55-57): hop h of bouquet b takes draws
+    b*8*depth + 8h + lane.  The oracle equals the brute-force restatement of
+    that order on the non-symmetric test graph (test_mt_walks) at depth 7;
+    here the restatement with the regular comparator's order (walk-major,
+    w*depth + h) gives different counts on a Chung-Lu graph at depth 6, so
+    the order is what the equality checks."""
+    from test_mt_walks import _mt_stream
+
+    g = G.chung_lu(60, 2.4, 5.0, seed=9)
+    depth, W = 6, 16
+
+    def brute(simd_order):
+        cnt = np.zeros(g.n, dtype=np.uint32)
+        for s in range(g.n):
+            draws = _mt_stream(42 + s, W * depth)
+            for w in range(W):
+                cur = s
+                for h in range(depth):
+                    o = (w // 8) * 8 * depth + 8 * h + w % 8 if simd_order else w * depth + h
+                    d = int(g.idx[cur + 1] - g.idx[cur])
+                    cur = int(g.adj[g.idx[cur] + int(draws[o]) % d])
+                    cnt[cur] += 1
+        return cnt
+
+    got = oracle.mt_walks(g.idx, g.adj, depth, W, variant=1)
+    assert np.array_equal(got, brute(True))
+    assert not np.array_equal(got, brute(False))
