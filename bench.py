@@ -112,6 +112,29 @@ def bench_saba(args, g=None, depth=None, steps=None, warmup=None):
     return value, ms_per_step, roof, cfg, g
 
 
+def gather_roof(mib: float):
+    """The binding roof of the walk's memory pattern, measured on this GPU by
+    tgtp_amd/bin/gather_roof (a child process): chains of dependent random
+    8-byte loads over a buffer of `mib` MiB, best over 1-8 waves per SIMD and
+    1-4 chains per lane.  Returns loads/s (None if the tool is missing)."""
+    import subprocess
+
+    tool = ROOT / "tgtp_amd" / "bin" / "gather_roof"
+    if not tool.exists():
+        return None
+    r = subprocess.run([str(tool), "--mib", f"{mib:.0f}", "--hops", "256", "--reps", "3"],
+                       capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        log(f"[bench] gather_roof failed: {r.stderr.strip()[:200]}")
+        return None
+    best = 0.0
+    for line in r.stdout.splitlines():
+        f = line.split()
+        if len(f) == 4 and f[0].isdigit():
+            best = max(best, float(f[2]))
+    return best or None
+
+
 def bench_tgtp(args):
     """tgtp() on the n=4096 Chung-Lu graph (omega=128, gamma=10): edges/s =
     CSR entries / seconds of one call with graph and eigenpairs resident in
@@ -264,20 +287,36 @@ def bench_batch(args):
     log(f"[bench] batch: {len(graphs)} graphs, sum n={sum(g.n for g in graphs)} "
         f"sum m={total_m}, generated in {time.time() - t0:.1f}s")
     t0 = time.time()
-    eig = [E.eigens_f32(*e) for e in E.compute_eigens_many(
-        graphs, 128, device="cuda:0",
-        progress=lambda done, tot: log(f"[bench] batch: eigenpairs {done}/{tot} "
-                                       f"({time.time() - t0:.1f}s)"))]
+    cache = Path(args.eig_cache) if args.eig_cache else None
+    if cache is not None and cache.exists():
+        # eigenpairs of an earlier run of this workload (profiling runs: no
+        # torch work in the profiled process)
+        with np.load(cache) as z:
+            lam_all, X_all = z["lam"], z["X"]
+        eig, off = [], 0
+        for k, g in enumerate(graphs):
+            eig.append((lam_all[k], np.ascontiguousarray(X_all[off:off + g.n])))
+            off += g.n
+    else:
+        eig = [E.eigens_f32(*e) for e in E.compute_eigens_many(
+            graphs, 128, device="cuda:0",
+            progress=lambda done, tot: log(f"[bench] batch: eigenpairs {done}/{tot} "
+                                           f"({time.time() - t0:.1f}s)"))]
+        if cache is not None:
+            np.savez(cache, lam=np.stack([lam for lam, _ in eig]),
+                     X=np.concatenate([X for _, X in eig]))
     t_eig = time.time() - t0
     log(f"[bench] batch: eigenpairs (batched block Krylov, FP64 on the GPU) in {t_eig:.1f}s")
     t0 = time.time()
     rng = np.random.default_rng(7)
     samples = [np.sort(rng.choice(g.m, size=min(args.batch_sample_edges, g.m), replace=False))
                for g in graphs]
-    exact = [exact_sc.sampled_scores(g, p, "cuda:0") for g, p in zip(graphs, samples)]
-    torch.cuda.empty_cache()
-    log(f"[bench] batch: exact resistances of {sum(len(p) for p in samples)} sampled edges "
-        f"in {time.time() - t0:.1f}s")
+    exact = None
+    if args.batch_exact:
+        exact = [exact_sc.sampled_scores(g, p, "cuda:0") for g, p in zip(graphs, samples)]
+        torch.cuda.empty_cache()
+        log(f"[bench] batch: exact resistances of {sum(len(p) for p in samples)} sampled edges "
+            f"in {time.time() - t0:.1f}s")
     out = {"workload": "256 Chung-Lu graphs " + WL.describe(
                {k: v for k, v in WL.BATCH256.items() if k != "count"}),
            "graphs": len(graphs), "sum_n": sum(g.n for g in graphs), "sum_m": total_m,
@@ -311,7 +350,8 @@ def bench_batch(args):
                 if not args.batch_plain:
                     plain_min = plain_max = float("nan")
                 t1 = time.perf_counter()
-                b = ctx.batch(graphs, eig, eps, 10, mode=mode, autotune=True, graph=True)
+                b = ctx.batch(graphs, eig, eps, 10, mode=mode, autotune=True,
+                              graph=bool(args.batch_graph))
                 t_create = time.perf_counter() - t1
                 outs, st = b.run()
                 times, kms = [], []
@@ -344,6 +384,8 @@ def bench_batch(args):
                 bit_equal = (all(np.array_equal(a.view(np.uint32), p.view(np.uint32))
                                  for a, p in zip(outs, plain_out)) if args.batch_plain else None)
                 est = np.concatenate([o[p] for o, p in zip(outs, samples)])
+                errs = (error_stats(est, np.concatenate(exact), eps) if exact is not None
+                        else None)
                 leg[name] = {"batched_edges_per_s": total_m / best, "batched_seconds": best,
                              "batched_seconds_max": worst, "batched_kernel_ms": min(kms),
                              "plain_edges_per_s": total_m / plain_min,
@@ -353,12 +395,11 @@ def bench_batch(args):
                              "timing": f"min (and max) of {reps} runs per side",
                              "create_autotune_capture_s": round(t_create, 2),
                              "autotuned": cfg, "bitwise_equal_plain": bit_equal,
-                             "vs_exact_fp64_sampled": error_stats(est, np.concatenate(exact),
-                                                                  eps)}
+                             "vs_exact_fp64_sampled": errs}
                 log(f"[bench] batch eps={eps} {name}: batched {total_m / best:.3e} edges/s "
                     f"({best * 1e3:.1f}-{worst * 1e3:.1f} ms), plain {total_m / plain_min:.3e} "
                     f"({plain_min * 1e3:.1f}-{plain_max * 1e3:.1f} ms), x{plain_min / best:.3f}, "
-                    f"cfg {cfg}, max err {leg[name]['vs_exact_fp64_sampled']['max_abs']:.4f}")
+                    f"cfg {cfg}, max err {errs['max_abs'] if errs else float('nan'):.4f}")
                 if mode == MODE_FAST_FP32 and args.batch_plain:
                     # the batch's small graphs alone, where per-call costs dominate
                     small = [i for i, g in enumerate(graphs) if g.n <= args.batch_small_n]
@@ -377,7 +418,7 @@ def bench_batch(args):
                         log(f"[bench] batch eps={eps} {name}: {len(small)} graphs with n <= "
                             f"{args.batch_small_n}: batched {sm / sb:.3e} edges/s, plain "
                             f"{sm / sp:.3e}, x{sp / sb:.2f}")
-            if not args.no_cpu:
+            if not args.no_cpu and exact is not None:
                 t1 = time.time()
                 leg["reference_algorithm_vs_exact_fp64_sampled"] = reference_algorithm_sampled(
                     graphs, eig, samples, exact, eps)
@@ -631,6 +672,12 @@ def main() -> int:
     ap.add_argument("--batch-small-n", type=int, default=8192)
     ap.add_argument("--batch-eps", default="0.05,0.01", help="comma-separated eps of the batch legs")
     ap.add_argument("--batch-modes", default="fp32,bf16", help="batch modes: fp32, bf16")
+    ap.add_argument("--batch-graph", type=int, default=1,
+                    help="0: run the batch uncaptured (counter collection runs)")
+    ap.add_argument("--batch-exact", type=int, default=1,
+                    help="0: skip the FP64 exact resistances of the sampled edges")
+    ap.add_argument("--eig-cache", default="",
+                    help="load the batch's eigenpairs from this .npz if it exists, else save them")
     ap.add_argument("--batch-plain", type=int, default=1,
                     help="0: skip the per-graph plain runs (profiling the batched run alone)")
     args = ap.parse_args()
@@ -650,6 +697,15 @@ def main() -> int:
         print(json.dumps(line), flush=True)
         return 0
     value, ms_per_step, roof, cfg, g = bench_saba(args)
+    roof_lps = gather_roof(g.m * 8 / 2**20)
+    if roof_lps:
+        roof["binding_roof"] = {
+            "what": "dependent random 8-byte loads over a buffer of the packed adjacency's "
+                    f"size ({g.m * 8 / 2**20:.0f} MiB), best over 1-8 waves/SIMD and 1-4 "
+                    "chains per lane (tgtp_amd/bin/gather_roof, measured in this run)",
+            "loads_per_s": roof_lps, "transitions_per_s": value, "frac": round(value / roof_lps, 4)}
+        log(f"[bench] random-gather roof {roof_lps:.3e} loads/s; walk at "
+            f"{value / roof_lps:.2f} of it")
     line = {"metric": METRIC, "value": value, "unit": "transitions/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",

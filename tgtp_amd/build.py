@@ -47,6 +47,10 @@ CLI_SOURCES = {"rw_experiment_saba": ("cli_rw.cpp", []),
                "aesc_example": ("aesc_example.cpp", [])}
 
 
+# stand-alone HIP measurement tools: binary -> source
+GPU_TOOLS = {"gather_roof": "gather_roof.hip"}
+
+
 def _run(cmd: list[str]) -> None:
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
@@ -89,6 +93,9 @@ def build(force: bool = True, verbose: bool = False) -> Path:
         clis.append(["g++", "-std=c++20", "-O2", *defines, f"-I{INCLUDE}", str(CSRC / src), "-o",
                      str(BIN / name), f"-L{PKG}", "-ltgtp_hip", f"-Wl,-rpath,{PKG}",
                      "-Wl,-rpath,$ORIGIN/..", "-fopenmp"])
+    # measurement tools (HIP, stand-alone)
+    for name, src in GPU_TOOLS.items():
+        clis.append([HIPCC, *HIP_FLAGS, str(CSRC / src), "-o", str(BIN / name)])
     with cf.ThreadPoolExecutor(max_workers=4) as ex:
         for f in [ex.submit(_run, j) for j in clis]:
             f.result()
