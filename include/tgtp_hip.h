@@ -88,6 +88,16 @@ enum {
  * the reference's rule in FAST modes; FAITHFUL always uses it. */
 #define TGTP_TAU_REFERENCE 0x100
 
+/* The 2-way walks of the reference's baseline tgtp (AESC::tgtp of
+ * aesc_baseline.h, the comparator driven by benchmark_baseline.cpp), a flag
+ * OR'ed into TGTP_MODE_FAITHFUL: instead of the SABA bouquets, each walk
+ * draws k = rndgen() % degree for the walker from s, then for the walker
+ * from v, at every hop, from one std::mt19937 seeded 42 at the start of the
+ * call and consumed in source and position order (aesc_baseline.h:17, 261,
+ * 363-379); each position's differences are summed in one float chain.
+ * Scores equal the reference baseline's bit for bit.  FAST modes refuse it. */
+#define TGTP_WALK2_MT19937 0x200
+
 /* AESCConfig (aesc.h:98-102); the reference CLI sets delta = 1/n
  * (main.cpp:36).  gamma >= 1, 0 < epsilon < 1. */
 typedef struct tgtp_params {

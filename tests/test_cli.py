@@ -150,6 +150,40 @@ def test_tgtp_cli_binary_caches(tmp_path):
     assert np.array_equal(h.idx, g.idx) and np.array_equal(h.adj, g.adj)
 
 
+def test_benchmark_baseline_cli_validation():
+    r = run([BIN / "benchmark_baseline"])
+    assert r.returncode == 1 and r.stderr.startswith("Usage: benchmark_baseline dataset")
+    r = run([BIN / "benchmark_baseline", "g", "0.05", "2"])
+    assert r.returncode == 1 and "baseline requires threads=1" in r.stderr
+
+
+@pytest.mark.gpu
+def test_benchmark_baseline_cli_checksum_equals_reference(tmp_path):
+    """benchmark_baseline (src/benchmark_baseline.cpp) on a golden fixture:
+    rows labelled mc2way-baseline whose checksum (the scores summed in order
+    in double, printed at setprecision(10)) equals that of the reference
+    build's own baseline scores."""
+    from conftest import load_golden
+
+    z = load_golden("chunglu300_w64_g1")
+    omega, gamma = int(z["omega"]), int(z["gamma"])
+    d = tmp_path / "cl300"
+    d.mkdir()
+    (d / "graph.txt").write_bytes(z["graph_txt"].tobytes())  # the text the reference read
+    E.write_eigens_txt(d / f"sorted_eigens_{omega}.txt", z["lam"], z["X"])
+    eps = [float(e) for e in z["epsilons"]]
+    r = run([BIN / "benchmark_baseline", "cl300", ",".join(repr(e) for e in eps), "1", "1",
+             tmp_path, str(omega), str(gamma), "0"])
+    assert r.returncode == 0, r.stderr
+    rows = [ln.split("\t") for ln in r.stdout.strip().splitlines()[1:]]
+    assert [x[0] for x in rows] == ["mc2way-baseline"] * len(eps)
+    for k, row in enumerate(rows):
+        acc = 0.0
+        for v in z[f"scores_mt_{k}"].astype(np.float64).tolist():
+            acc += v
+        assert row[6] == f"{acc:.10g}", (k, row[6])
+
+
 @pytest.mark.gpu
 def test_benchmark_cli_rows(tmp_path):
     g = G.chung_lu(600, 2.5, 9.0, seed=13)

@@ -13,7 +13,7 @@ import pytest
 
 import oracle
 from conftest import GOLDEN_CASES, load_golden
-from tgtp_amd import MODE_FAITHFUL, TgtpError
+from tgtp_amd import MODE_FAITHFUL, MODE_FAST_FP32, WALK2_MT19937, TgtpError
 from tgtp_amd import eigenprep as E
 from tgtp_amd import graph as G
 from tgtp_amd import workloads as W
@@ -78,6 +78,39 @@ def test_faithful_chunglu4096_bitwise(gpu_ctx, eps):
     _say(f"  kernel_ms={st['kernel_ms']:.2f} taus_ms={st['taus_ms']:.2f} "
          f"diffuse_ms={st['diffuse_ms']:.2f} host_fixups={st['host_fixups']} "
          f"exact_selections={st['work_items']}")
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_mt19937_walks_equal_reference_baseline(gpu_ctx, name):
+    """The reference's baseline tgtp (aesc_baseline.h: one std::mt19937 seeded
+    42, draws in source and position order, one float chain per position):
+    scores equal the reference build's own outputs bit for bit."""
+    z = load_golden(name)
+    g = G.Graph(z["idx"], z["adj"])
+    for k, eps in enumerate(z["epsilons"]):
+        _say(f"mt19937 walks {name}: eps={float(eps)}")
+        s, st = gpu_ctx.score(g, z["lam"], z["X"], float(eps), int(z["gamma"]),
+                              mode=MODE_FAITHFUL | WALK2_MT19937)
+        ref = z[f"scores_mt_{k}"]
+        bad = np.flatnonzero(s.view(np.uint32) != ref.view(np.uint32))
+        assert bad.size == 0, f"{bad.size} scores differ, first at {bad[:5]}"
+
+
+@pytest.mark.parametrize("eps", [0.05, 0.01])
+def test_mt19937_walks_chunglu4096_equal_oracle(gpu_ctx, eps):
+    g = W.chung_lu_4096()
+    lam, X = E.eigens_f32(*E.compute_eigens(g, 128))
+    s, st = gpu_ctx.score(g, lam, X, eps, 10, mode=MODE_FAITHFUL | WALK2_MT19937)
+    ref = oracle.tgtp(g.idx, g.adj, lam, X, eps, 10, walk_mode=oracle.WALK_MT19937)
+    assert np.array_equal(s.view(np.uint32), ref.view(np.uint32))
+    _say(f"  mt19937 walks eps={eps}: kernel_ms={st['kernel_ms']:.2f}")
+
+
+def test_mt19937_walks_refused_outside_faithful(gpu_ctx):
+    g = G.cycle(15)
+    lam, X = E.eigens_f32(*E.compute_eigens(g, 6))
+    with pytest.raises(TgtpError, match="mode"):
+        gpu_ctx.score(g, lam, X, 0.05, 10, mode=MODE_FAST_FP32 | WALK2_MT19937)
 
 
 def test_faithful_symmetric_graphs_tie_selection(gpu_ctx):

@@ -9,7 +9,7 @@ Layers:
   tgtp_amd/bin/{tgtp,benchmark,rw_experiment_saba,rw_experiment,rw_experiment_simd}
                                                  command-line drop-ins
 """
-from ._lib import (MODE_FAITHFUL, MODE_FAST_BF16, MODE_FAST_FP32, TAU_REFERENCE,  # noqa: F401
+from ._lib import (MODE_FAITHFUL, MODE_FAST_BF16, MODE_FAST_FP32, TAU_REFERENCE, WALK2_MT19937,  # noqa: F401,E501
                    WALK_MT_REGULAR, WALK_MT_SIMD, TgtpError)
 from .graph import (Graph, chung_lu, complete, cycle, grid_rewired, load_eigens_txt,  # noqa: F401
                     load_eigens_txt_cached, load_txt, load_txt_cached, petersen)

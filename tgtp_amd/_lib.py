@@ -25,6 +25,8 @@ MODE_FAST_FP32 = 1
 MODE_FAST_BF16 = 2
 # FAST modes: the reference's tau refinement rule (aesc.h:130) instead of the bounded one
 TAU_REFERENCE = 0x100
+# FAITHFUL with the MT19937 2-way walks of the reference baseline (aesc_baseline.h:363-379)
+WALK2_MT19937 = 0x200
 
 WALK_MT_REGULAR = 0
 WALK_MT_SIMD = 1

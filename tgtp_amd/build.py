@@ -44,6 +44,7 @@ CLI_SOURCES = {"rw_experiment_saba": ("cli_rw.cpp", []),
                "rw_experiment": ("cli_rw.cpp", ["-DTGTP_RW_VARIANT=1"]),
                "rw_experiment_simd": ("cli_rw.cpp", ["-DTGTP_RW_VARIANT=2"]),
                "tgtp": ("cli_tgtp.cpp", []), "benchmark": ("cli_benchmark.cpp", []),
+               "benchmark_baseline": ("cli_benchmark.cpp", ["-DTGTP_BENCH_BASELINE=1"]),
                "aesc_example": ("aesc_example.cpp", [])}
 
 

@@ -6,6 +6,11 @@
 //
 // usage: benchmark dataset [eps_csv=0.05,0.01,0.005] [threads_csv=1]
 //                  [repeats=3] [data_root=datasets] [omega=128] [gamma=10] [warmups=1]
+//
+// Built with TGTP_BENCH_BASELINE=1 it is the benchmark_baseline drop-in
+// (src/benchmark_baseline.cpp): the reference's baseline tgtp with the
+// MT19937 2-way walks (TGTP_MODE_FAITHFUL | TGTP_WALK2_MT19937), threads
+// must be 1, rows labelled mc2way-baseline.
 #include <chrono>
 #include <cmath>
 #include <filesystem>
@@ -15,15 +20,25 @@
 
 #include "cli_common.hpp"
 
+#ifndef TGTP_BENCH_BASELINE
+#define TGTP_BENCH_BASELINE 0
+#endif
+
 int main(int argc, char** argv) {
   if (argc < 2 || argc > 9) {
-    std::cerr << "Usage: benchmark dataset [eps_csv=0.05,0.01,0.005] [threads_csv=1]"
-              << " [repeats=3] [data_root=datasets] [omega=128] [gamma=10] [warmups=1]\n";
+    if (TGTP_BENCH_BASELINE)
+      std::cerr << "Usage: benchmark_baseline dataset [eps_csv=0.05,0.01,0.005] [threads=1]"
+                << " [repeats=3] [data_root=datasets] [omega=128] [gamma=10] [warmups=1]\n";
+    else
+      std::cerr << "Usage: benchmark dataset [eps_csv=0.05,0.01,0.005] [threads_csv=1]"
+                << " [repeats=3] [data_root=datasets] [omega=128] [gamma=10] [warmups=1]\n";
     return 1;
   }
   try {
     const auto eps = cli::split_csv(argc > 2 ? argv[2] : "0.05,0.01,0.005");
     const auto threads = cli::split_csv(argc > 3 ? argv[3] : "1");
+    if (TGTP_BENCH_BASELINE && (threads.size() != 1 || cli::positive(threads[0]) != 1))
+      throw std::invalid_argument("baseline requires threads=1");
     const uint32_t repeats = cli::positive(argc > 4 ? argv[4] : "3");
     const double warmup_arg = cli::number(argc > 8 ? argv[8] : "1");
     if (warmup_arg < 0 || warmup_arg > INT32_MAX || warmup_arg != std::floor(warmup_arg))
@@ -51,7 +66,9 @@ int main(int argc, char** argv) {
     tgtp_dgraph* dg = nullptr;
     ctx.check(tgtp_graph_upload(ctx.c, &gv, &dg), "tgtp_graph_upload");
     const tgtp_eigens ev{omega, lam.data(), X.data()};
-    const int32_t mode = cli::mode_from_env();
+    const int32_t mode =
+        TGTP_BENCH_BASELINE ? (TGTP_MODE_FAITHFUL | TGTP_WALK2_MT19937) : cli::mode_from_env();
+    const char* sampler = TGTP_BENCH_BASELINE ? "mc2way-baseline" : "seed32-grid-multiply-high";
     std::vector<float> scores(csr.g.m);
     std::cout << std::setprecision(10) << std::unitbuf;
     std::cout << "sampler\tdataset\tepsilon\tthreads\trepeat\tseconds\tchecksum\n";
@@ -69,8 +86,8 @@ int main(int argc, char** argv) {
             checksum += scores[j];
           }
           if (run >= warmups)
-            std::cout << "seed32-grid-multiply-high" << '\t' << dataset << '\t' << pv.epsilon << '\t'
-                      << t << '\t' << run - warmups << '\t' << elapsed << '\t' << checksum << '\n';
+            std::cout << sampler << '\t' << dataset << '\t' << pv.epsilon << '\t' << t << '\t'
+                      << run - warmups << '\t' << elapsed << '\t' << checksum << '\n';
         }
       }
     }

@@ -79,7 +79,7 @@ struct tgtp_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev[8] = {};
   std::string err;
-  DevBuf bufs[16];
+  DevBuf bufs[24];
   void* scratch = nullptr;  // this device's copy of capi.hip's static scratch (no-allocation entry)
 };
 
@@ -100,6 +100,9 @@ enum BufSlot {
   kBufWalkRec = 12,
   kBufWalkFill = 13,
   kBufSubset = 14,  // source list of tgtp_score_sources
+  kBufMtEll = 15,   // MT19937 2-way walks: rounds per source
+  kBufMtOff = 16,   // ... stream offset per CSR position
+  kBufMtDraws = 17, // ... the stream
 };
 
 // Returns a device buffer of at least `bytes` in slot `slot` (grow-only).
@@ -126,6 +129,8 @@ int launch_walk_counter_sums(tgtp_ctx* ctx, const uint32_t* d_counters, uint64_t
 int launch_mt_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin, uint32_t src_end,
                     uint32_t depth, uint32_t n_walks, int32_t variant, uint32_t* d_counters,
                     unsigned long long* d_sums, float* walk_ms);
+// The first `count` outputs of std::mt19937(seed), in order (mt_walk.hip).
+int launch_mt_stream(tgtp_ctx* ctx, uint32_t seed, uint64_t count, uint32_t* d_out);
 int launch_saba_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,
                       uint32_t src_end, uint32_t depth, uint32_t n_walks,
                       uint32_t* d_counters, unsigned long long* d_sums,

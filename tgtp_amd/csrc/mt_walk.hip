@@ -145,7 +145,28 @@ __global__ void __launch_bounds__(kMtBlock) mt_walk_kernel(MtWalkArgs a) {
     if (hot[v]) atomicAdd(&a.counts[v], hot[v]);
 }
 
+// One std::mt19937 stream seeded `seed`: outputs [0, count) in order, one
+// workgroup (the generator is sequential; the twist is spread over its lanes).
+__global__ void __launch_bounds__(kMtBlock) mt_stream_kernel(uint32_t seed, uint64_t count,
+                                                           uint32_t* __restrict__ out) {
+  __shared__ uint32_t mt[kMtN];
+  mt_seed(mt, seed);
+  for (uint64_t base = 0; base < count; base += kMtN) {
+    mt_twist(mt);
+    for (uint32_t i = threadIdx.x; i < kMtN; i += kMtBlock)
+      if (base + i < count) out[base + i] = mt_temper(mt[i]);
+    __syncthreads();
+  }
+}
+
 }  // namespace
+
+int launch_mt_stream(tgtp_ctx* ctx, uint32_t seed, uint64_t count, uint32_t* d_out) {
+  if (count == 0) return TGTP_OK;
+  mt_stream_kernel<<<1, kMtBlock, 0, ctx->stream>>>(seed, count, d_out);
+  TGTP_HIP_CHECK(ctx, hipGetLastError());
+  return TGTP_OK;
+}
 
 int launch_mt_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin, uint32_t src_end,
                     uint32_t depth, uint32_t n_walks, int32_t variant, uint32_t* d_counters,

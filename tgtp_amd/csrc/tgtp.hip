@@ -215,6 +215,13 @@ struct SrcArgs {
   uint32_t* status;   // bit 0: a walk count exceeded kMaxWalks
   const uint32_t* src_list;  // sources to run (null: all n, in id order)
   uint32_t n_src;            // entries of src_list
+  // 2-way walks: 0 SABA (aesc.h:241-391); with the MT19937 comparator
+  // (aesc_baseline.h:363-379) 1 = counting pass (walk counts only, no
+  // walks), 2 = walk pass reading each position's draws from mt_draws at
+  // mt_off[p]
+  int32_t walk_mode;
+  const uint32_t* mt_draws;
+  const uint64_t* mt_off;
   uint32_t n;
   int32_t gamma;
   double eps;
@@ -473,9 +480,38 @@ __device__ void faithful_source(const SrcArgs& A, uint32_t src, float* pvec, uin
   }
   __syncthreads();
 
+  if (A.walk_mode == 2) {
+    // ---- MT19937 2-way walks of the comparator (aesc_baseline.h:363-379):
+    // position p's walks consume draws mt_off[p] .. in the order the single
+    // global generator hands them out (walk i, hop len: the walker from src,
+    // then the walker from v); each position's differences are added in one
+    // float chain, so a lane takes a whole position
+    for (uint32_t p = r0 + ln; p < r1; p += 64) {
+      const int rest = A.taus[p] - ell;
+      if (rest <= 0) continue;
+      const uint64_t nr = A.nr_pos[p];
+      const uint32_t v = A.adj[p];
+      const uint32_t* d = A.mt_draws + A.mt_off[p];
+      float x = 0.0f;
+      for (uint64_t i = 0; i < nr; ++i) {
+        uint32_t c1 = src, c2 = v;
+        for (int len = 0; len < rest; ++len) {
+          const uint2 d1 = A.rowd[c1];
+          c1 = A.adj[d1.x + d[0] % d1.y];
+          const uint2 d2 = A.rowd[c2];
+          c2 = A.adj[d2.x + d[1] % d2.y];
+          d += 2;
+          x += (pvec[c1] - pvec[c2]);
+        }
+      }
+      A.preds[p] += x / (ds * static_cast<float>(nr));
+      if (A.tr_x) A.tr_x[p] = x;
+    }
+  }
+
   // ---- 2-way SABA walks (aesc.h:241-391), eight positions per wave
   const uint32_t grp = ln >> 3, b = ln & 7u;
-  for (uint32_t pb = r0; pb < r1; pb += 8) {
+  for (uint32_t pb = r0; pb < r1 && A.walk_mode == 0; pb += 8) {
     const uint32_t p = pb + grp;
     uint64_t nr = 0;
     uint32_t L = 0, v = 0;
@@ -869,11 +905,60 @@ static int run_score_faithful(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens*
     fill_i32_kernel<<<grid_for(5 * n + m, 256, 4096), 256, 0, s>>>(
         reinterpret_cast<int32_t*>(tr_buf), 5 * n + m, 0);
   }
-  if (use_lds)
-    source_faithful_kernel<true><<<slots, 64, lds_bytes, s>>>(A);
-  else
-    source_faithful_kernel<false><<<slots, 64, 0, s>>>(A);
-  TGTP_HIP_CHECK(ctx, hipGetLastError());
+  auto launch_sources = [&]() -> int {
+    if (use_lds)
+      source_faithful_kernel<true><<<slots, 64, lds_bytes, s>>>(A);
+    else
+      source_faithful_kernel<false><<<slots, 64, 0, s>>>(A);
+    TGTP_HIP_CHECK(ctx, hipGetLastError());
+    return TGTP_OK;
+  };
+  if (prm->mode & TGTP_WALK2_MT19937) {
+    // the comparator's walks draw from one generator in source and position
+    // order (aesc_baseline.h:261, 338): a counting pass gives every
+    // position's walk count and rest, their prefix sums give each
+    // position's offset in the stream, which is generated once; the walk
+    // pass then repeats the (deterministic) per-source pipeline with walks
+    auto* d_ell = static_cast<int32_t*>(ctx_buffer(ctx, kBufMtEll, n * 4 + 64));
+    if (!d_ell) return set_error(ctx, TGTP_E_NOMEM, "walk rounds buffer");
+    int32_t* keep_ell = A.tr_ell;
+    A.tr_ell = keep_ell ? keep_ell : d_ell;
+    A.walk_mode = 1;
+    if ((rc = launch_sources())) return rc;
+    std::vector<uint64_t> nr_h(m), off(m + 1, 0);
+    std::vector<int32_t> tau_h(m), ell_h(n);
+    TGTP_HIP_CHECK(ctx, hipMemcpyAsync(nr_h.data(), d_nr, m * 8, hipMemcpyDeviceToHost, s));
+    TGTP_HIP_CHECK(ctx, hipMemcpyAsync(tau_h.data(), d_taus, m * 4, hipMemcpyDeviceToHost, s));
+    TGTP_HIP_CHECK(ctx, hipMemcpyAsync(ell_h.data(), A.tr_ell, n * 4, hipMemcpyDeviceToHost, s));
+    uint32_t w0[3] = {0, 0, 0};
+    TGTP_HIP_CHECK(ctx, hipMemcpyAsync(w0, d_words, 12, hipMemcpyDeviceToHost, s));
+    TGTP_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    if (w0[2] & 1u)
+      return set_error(ctx, TGTP_E_UNSUPPORTED,
+                       "a walk count exceeded 2^36 walks per edge; scores are not computed");
+    const uint32_t* hidx = P->h_idx.data();
+    for (uint64_t u = 0; u < n; ++u)
+      for (uint32_t p = hidx[u]; p < hidx[u + 1]; ++p) {
+        const int rest = tau_h[p] - ell_h[u];
+        off[p + 1] = off[p] + (rest > 0 ? 2ull * nr_h[p] * static_cast<uint64_t>(rest) : 0ull);
+      }
+    const uint64_t draws = off[m];
+    if (draws > (1ull << 32))
+      return set_error(ctx, TGTP_E_UNSUPPORTED,
+                       "the MT19937 walks need more than 2^32 draws (16 GiB of stream)");
+    auto* d_off = static_cast<uint64_t*>(ctx_buffer(ctx, kBufMtOff, (m + 1) * 8));
+    auto* d_draws = static_cast<uint32_t*>(ctx_buffer(ctx, kBufMtDraws, draws * 4 + 64));
+    if (!d_off || !d_draws) return set_error(ctx, TGTP_E_NOMEM, "MT19937 stream");
+    TGTP_HIP_CHECK(ctx, hipMemcpyAsync(d_off, off.data(), (m + 1) * 8, hipMemcpyHostToDevice, s));
+    if ((rc = launch_mt_stream(ctx, 42u, draws, d_draws))) return rc;  // rndgen.seed(42)
+    fill_words_kernel<<<1, 64, 0, s>>>(d_words, 16);
+    A.walk_mode = 2;
+    A.mt_draws = d_draws;
+    A.mt_off = d_off;
+    A.tr_ell = keep_ell;
+    TGTP_HIP_CHECK(ctx, hipStreamSynchronize(s));  // host vectors above stay alive until here
+  }
+  if ((rc = launch_sources())) return rc;
   TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[6], s));
   symmetrize_kernel<<<grid_for(m, 256, 4096), 256, 0, s>>>(d_preds, P->d_src_of, m, d_out);
   TGTP_HIP_CHECK(ctx, hipGetLastError());
@@ -968,6 +1053,8 @@ extern "C" int tgtp_score_sources(tgtp_ctx* ctx, const tgtp_dgraph* cdg, const t
   auto* dg = const_cast<tgtp_dgraph*>(cdg);
   int rc = validate_params(ctx, p);
   if (rc) return rc;
+  if (p->mode & TGTP_WALK2_MT19937)  // its stream offsets depend on every earlier source
+    return set_error(ctx, TGTP_E_ARG, "source subsets do not take the MT19937 walks");
   std::vector<std::pair<uint32_t, uint32_t>> rg(n_ranges);
   uint64_t count = 0;
   for (uint32_t k = 0; k < n_ranges; ++k) {

@@ -29,11 +29,13 @@ struct TraceOut {
   uint32_t* q_size;
 };
 
-// tgtp_params.mode = TGTP_MODE_* | optional TGTP_TAU_REFERENCE
+// tgtp_params.mode = TGTP_MODE_* | optional TGTP_TAU_REFERENCE (FAST) or
+// TGTP_WALK2_MT19937 (FAITHFUL)
 inline int32_t tgtp_mode_base(int32_t mode) { return mode & 0xff; }
 inline bool tgtp_mode_valid(int32_t mode) {
   const int32_t b = tgtp_mode_base(mode);
-  return (mode & ~(0xff | TGTP_TAU_REFERENCE)) == 0 &&
+  if ((mode & TGTP_WALK2_MT19937) && b != TGTP_MODE_FAITHFUL) return false;
+  return (mode & ~(0xff | TGTP_TAU_REFERENCE | TGTP_WALK2_MT19937)) == 0 &&
          (b == TGTP_MODE_FAITHFUL || b == TGTP_MODE_FAST_FP32 || b == TGTP_MODE_FAST_BF16);
 }
 
