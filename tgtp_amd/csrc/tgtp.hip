@@ -257,20 +257,68 @@ __device__ void faithful_source(const SrcArgs& A, uint32_t src, float* pvec, uin
   const double dse2 = dse_d * dse_d;  // pow(ds * eps, 2), aesc.h:498
   __syncthreads();
 
+  // Ordering of the push's state between frontier vertices.  pvec / seen in
+  // LDS: the workgroup is one wave and the LDS operations of a wave are
+  // performed in order, so a compiler-level wave barrier keeps the program
+  // order and no wait is needed (the prefetch loads below stay in flight).
+  // pvec / seen in global memory: a full barrier, so that each store has
+  // completed before a later load of the same word.
+  auto vsync = [&]() {
+    if constexpr (kLds) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+      __syncthreads();
+    }
+  };
+
   for (;;) {
     // ---- diffuse: aesc.h:222-234, frontier in order
     uint32_t qn_size = 0;
+    // The queue, row descriptors, adjacency and receiver degrees do not
+    // depend on pvec: a four-stage pipeline loads, for frontier entries
+    // i+4 .. i+1, the vertex id, its row descriptor, this lane's neighbour in
+    // the row's first 64 entries, and that neighbour's degree, one stage per
+    // entry and iteration, so no entry waits on its own chain of loads.
+    // Entries past the frontier repeat the source (a valid vertex).
+    auto qv = [&](uint32_t k) { return k < qsize ? q[k] : src; };
+    auto first_nb = [&](uint2 r) { return ln < r.y ? A.adj[r.x + ln] : 0xFFFFFFFFu; };
+    auto nb_deg = [&](uint32_t u) { return u != 0xFFFFFFFFu ? A.degf[u] : 1.0f; };
+    uint32_t V1 = qv(1), V2 = qv(2), V3 = qv(3);
+    uint2 R0 = A.rowd[qv(0)], R1 = A.rowd[V1], R2 = A.rowd[V2];
+    uint32_t V0 = qv(0);
+    uint32_t U0 = first_nb(R0), U1 = first_nb(R1);
+    float G0 = nb_deg(U0);
     for (uint32_t i = 0; i < qsize; ++i) {
-      const uint32_t v = q[i];
-      const uint2 rv = A.rowd[v];
+      const uint32_t v = V0;
+      const uint2 rv = R0;
+      const uint32_t u_first = U0;
+      const float g_first = G0;
+      {  // next entries' stages
+        const float G1 = nb_deg(U1);
+        const uint32_t U2 = first_nb(R2);
+        const uint2 R3 = A.rowd[V3];
+        const uint32_t V4 = qv(i + 4);
+        V0 = V1;
+        V1 = V2;
+        V2 = V3;
+        V3 = V4;
+        R0 = R1;
+        R1 = R2;
+        R2 = R3;
+        U0 = U1;
+        U1 = U2;
+        G0 = G1;
+      }
       const float residue = pvec[v];
-      __syncthreads();
+      vsync();
       if (ln == 0) pvec[v] = 0.0f;
-      __syncthreads();
+      vsync();
       for (uint32_t base = 0; base < rv.y; base += 64) {
         const uint32_t j = base + ln;
         const bool valid = j < rv.y;
-        const uint32_t u = valid ? A.adj[rv.x + j] : 0xFFFFFFFFu;
+        const uint32_t u = base == 0 ? u_first : (valid ? A.adj[rv.x + j] : 0xFFFFFFFFu);
         const uint32_t up = __shfl_up(u, 1);
         const bool lead = valid && (ln == 0 || u != up);
         bool newrun = lead;
@@ -291,14 +339,16 @@ __device__ void faithful_source(const SrcArgs& A, uint32_t src, float* pvec, uin
         }
         qn_size += __popcll(fm);
         if (lead) {
-          const float c = residue / A.degf[u];
+          const float c = residue / (base == 0 ? g_first : A.degf[u]);
           float t = pvec[u];
           for (uint32_t k = 0; k < run; ++k) t = t + c;
           pvec[u] = t;
         }
-        __syncthreads();
+        vsync();
       }
     }
+    // the next queue (global memory) is read back below: its stores complete
+    __syncthreads();
     // preds update (aesc.h:235-237)
     const float ps = pvec[src];
     for (uint32_t p = r0 + ln; p < r1; p += 64) A.preds[p] += (ps - pvec[A.adj[p]]) / ds;
