@@ -38,8 +38,11 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_TRANSITION = 20  # row bounds 8 + neighbour 4 + counter RMW 8
 
 
+T_START = time.perf_counter()
+
+
 def log(msg: str) -> None:
-    print(msg, file=sys.stderr, flush=True)
+    print(f"{msg} [t={time.perf_counter() - T_START:.0f}s]", file=sys.stderr, flush=True)
 
 
 def host_threads() -> int:

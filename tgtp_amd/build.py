@@ -37,7 +37,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-std=c++20", "-O2", "-fPIC", "-fopenmp", "-ffp-contract=off",
               f"-I{INCLUDE}", f"-I{CSRC}"]
 
-HIP_SOURCES = ["capi.hip", "saba_walk.hip", "mt_walk.hip", "tgtp.hip", "tgtp_fast.hip", "fast_block.hip", "taus_fast.hip", "tgtp_batch.hip"]
+HIP_SOURCES = ["capi.hip", "saba_walk.hip", "saba_part.hip", "mt_walk.hip", "tgtp.hip", "tgtp_fast.hip", "fast_block.hip", "taus_fast.hip", "tgtp_batch.hip"]
 HOST_SOURCES = ["graph_io.cpp", "host_tgtp.cpp"]
 # binary -> (source, extra defines)
 CLI_SOURCES = {"rw_experiment_saba": ("cli_rw.cpp", []),

@@ -54,6 +54,11 @@ struct tgtp_dgraph {
   // computed once per (hot, bshift) from the row degrees
   std::vector<double> walk_mass;
   uint32_t walk_mass_hot = 0, walk_mass_bshift = 0;
+  // vertex partitions of the partitioned walk engine (saba_part.hip): bounds
+  // on the host, bounds and the coarse lookup table on the device
+  std::vector<uint32_t> part_plo;
+  uint32_t part_vmax = 0, part_want = 0, part_tshift = 0;
+  void* part_dev = nullptr;
   // tgtp host-side state (host CSR copy, symmetrization map), built lazily
   void* prep = nullptr;
   void (*prep_free)(void*) = nullptr;
@@ -103,6 +108,11 @@ enum BufSlot {
   kBufMtEll = 15,   // MT19937 2-way walks: rounds per source
   kBufMtOff = 16,   // ... stream offset per CSR position
   kBufMtDraws = 17, // ... the stream
+  kBufPartRecA = 18,   // partitioned walks: record pools of two rounds
+  kBufPartRecB = 19,
+  kBufPartMeta = 20,   // ... their block tables
+  kBufPartCopies = 21, // ... per-workgroup counter copies
+  kBufPartTaint = 22,  // ... bouquet fallback bitmap
 };
 
 // Returns a device buffer of at least `bytes` in slot `slot` (grow-only).
@@ -131,6 +141,13 @@ int launch_mt_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin, ui
                     unsigned long long* d_sums, float* walk_ms);
 // The first `count` outputs of std::mt19937(seed), in order (mt_walk.hip).
 int launch_mt_stream(tgtp_ctx* ctx, uint32_t seed, uint64_t count, uint32_t* d_out);
+// Partitioned walk engine (saba_part.hip) for large graphs: plans the
+// partitions (cached on the graph) and reports whether a pass of these
+// parameters takes it; then runs a pass, adding into cleared counters.
+int saba_part_prepare(tgtp_ctx* ctx, tgtp_dgraph* dg, uint64_t n_src, uint32_t depth,
+                      uint32_t n_walks, bool* use);
+int launch_saba_walks_part(tgtp_ctx* ctx, tgtp_dgraph* dg, uint32_t src_begin, uint64_t n_src,
+                           uint32_t depth, uint32_t n_walks, uint32_t* d_counters);
 int launch_saba_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,
                       uint32_t src_end, uint32_t depth, uint32_t n_walks,
                       uint32_t* d_counters, unsigned long long* d_sums,

@@ -450,16 +450,24 @@ int launch_saba_walks(tgtp_ctx* ctx, const tgtp_dgraph* cdg, uint32_t src_begin,
   auto* dg = const_cast<tgtp_dgraph*>(cdg);
   hipStream_t s = ctx->stream;
   const uint64_t n = dg->n;
-  bool packed = false;
-  int rc = ensure_walk_adj(ctx, dg, &packed);
+  const uint64_t n_src = src_end > src_begin ? src_end - src_begin : 0;
+  const uint32_t bps = n_walks / 8u;
+  const bool any = n_src > 0 && bps > 0 && depth > 0 && dg->m > 0;  // m == 0: no vertex can step
+  // large graphs: the partitioned engine (saba_part.hip), planned before the
+  // timed window like the packed layout
+  bool part = false;
+  int rc = any ? saba_part_prepare(ctx, dg, n_src, depth, n_walks, &part) : TGTP_OK;
   if (rc) return rc;
+  bool packed = false;
+  if (!part && (rc = ensure_walk_adj(ctx, dg, &packed))) return rc;
   TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], s));
   if ((rc = launch_walk_counters_clear(ctx, d_counters, n, d_sums))) return rc;
   TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], s));
 
-  const uint64_t n_src = src_end > src_begin ? src_end - src_begin : 0;
-  const uint32_t bps = n_walks / 8u;
-  if (n_src > 0 && bps > 0 && depth > 0 && dg->m > 0) {  // m == 0: no vertex can step
+  if (any && part) {
+    if ((rc = launch_saba_walks_part(ctx, dg, src_begin, n_src, depth, n_walks, d_counters)))
+      return rc;
+  } else if (any) {
     WalkParams P{};
     P.src_begin = src_begin;
     P.bps = bps;
