@@ -56,14 +56,17 @@ int env_int(const char* name, int dflt) {
   return s && *s ? std::atoi(s) : dflt;
 }
 
+#ifndef TGTP_PART_PER
+#define TGTP_PART_PER 6
+#endif
 constexpr int kPBlock = 1024;                // threads per workgroup
-constexpr uint32_t kBR = 2048;               // records per block = per tile
-constexpr uint32_t kPer = kBR / kPBlock;     // records per thread per tile
+constexpr uint32_t kPer = TGTP_PART_PER;     // records per thread per tile
+constexpr uint32_t kBR = kPBlock * kPer;     // records per block = per tile
 constexpr uint32_t kMaxParts = 256;          // partitions (= sort bins) at most
 constexpr uint32_t kTab = 2048;              // coarse vertex -> partition table
-constexpr uint32_t kList = 1024;             // block ids of a partition staged in LDS at a time
+constexpr uint32_t kList = 512;              // block ids of a partition staged in LDS at a time
 constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr uint32_t kMaxPoolBlocks = (1u << 21) - 1;  // record index fits 32 bits
+constexpr uint32_t kMaxPoolBlocks = 0xFFFFFFFFu / kBR;  // record index fits 32 bits
 
 enum Mode { kGen = 0, kMid = 1, kLast = 2 };
 
@@ -75,11 +78,13 @@ enum Mode { kGen = 0, kMid = 1, kLast = 2 };
 // block per partition), so no block is taken with a global atomic.
 struct PartRound {
   const uint2* rowd;
+  const uint32_t* idx;
   const uint32_t* adj;
   const uint32_t* plo;   // [P+1] partition vertex bounds
   const uint16_t* ptab;  // [kTab+2]
   uint64_t n;
   uint32_t P;
+  uint32_t vbig;         // vertices of the largest partition
   uint32_t tshift;
   uint32_t depth;
   uint32_t round;        // hop performed by this launch (0-based); kLast: depth
@@ -194,14 +199,26 @@ __device__ __forceinline__ void scan_bins(PartLds& L) {
 
 // One tile: each thread holds up to kPer walkers (valid[j]) at vertex cur[j]
 // that take hop `round`.  Sorts the successors by partition and appends them
-// to the workgroup's blocks.
+// to the workgroup's blocks.  kLdsRows: the walkers are in the partition whose
+// row offsets ridx[0, V] (vertices lo + i) are staged in LDS, so the hop's only
+// gather is the neighbour itself; otherwise the row descriptors are gathered.
+template <bool kLdsRows>
 __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A,
+                                                 const uint32_t* ridx, uint32_t lo,
                                                  uint32_t (&cur)[kPer], uint32_t (&sec)[kPer],
                                                  uint32_t (&wid)[kPer], bool (&valid)[kPer]) {
   const uint32_t tid = threadIdx.x;
   uint2 rd[kPer];
 #pragma unroll
-  for (uint32_t j = 0; j < kPer; ++j) rd[j] = valid[j] ? A.rowd[cur[j]] : make_uint2(0u, 0u);
+  for (uint32_t j = 0; j < kPer; ++j) {
+    if constexpr (kLdsRows) {
+      const uint32_t a = valid[j] ? ridx[cur[j] - lo] : 0u;
+      const uint32_t b = valid[j] ? ridx[cur[j] - lo + 1u] : 0u;
+      rd[j] = make_uint2(a, b - a);
+    } else {
+      rd[j] = valid[j] ? A.rowd[cur[j]] : make_uint2(0u, 0u);
+    }
+  }
   uint32_t nxt[kPer], dest[kPer], rank[kPer];
 #pragma unroll
   for (uint32_t j = 0; j < kPer; ++j) {
@@ -306,11 +323,13 @@ __device__ __forceinline__ void load_tile(const PartRound& A, uint32_t blk, uint
 }
 
 template <int kMode>
-__global__ void __launch_bounds__(kPBlock, 2)
+__global__ void __launch_bounds__(kPBlock)
 part_round_kernel(PartRound A) {
   extern __shared__ __align__(16) uint8_t lds_raw[];
   PartLds& L = *reinterpret_cast<PartLds*>(lds_raw);
+  // the current partition's arrival counters and row offsets
   uint32_t* cnt = reinterpret_cast<uint32_t*>(lds_raw + sizeof(PartLds));
+  uint32_t* ridx = cnt + A.vbig;
   const uint32_t tid = threadIdx.x;
   const uint32_t w = blockIdx.x;
   const uint32_t xc = w & 7u;
@@ -371,7 +390,7 @@ part_round_kernel(PartRound A) {
         sec[j] = (i + 1u) * A.step;
         if (valid[j] && sec[j] == 0u) mark_bouquet(A.taint, wid[j]);
       }
-      step_and_scatter(L, A, cur, sec, wid, valid);
+      step_and_scatter<false>(L, A, nullptr, 0u, cur, sec, wid, valid);
     }
   } else {
     uint32_t* copy = A.copies + static_cast<uint64_t>(l) * A.n;
@@ -381,6 +400,8 @@ part_round_kernel(PartRound A) {
       const uint32_t lo = L.plo[p];
       const uint32_t V = L.plo[p + 1] - lo;
       for (uint32_t i = tid; i < V; i += kPBlock) cnt[i] = 0u;
+      if constexpr (kMode == kMid)
+        for (uint32_t i = tid; i <= V; i += kPBlock) ridx[i] = A.idx[lo + i];
       const uint32_t base = A.pstart_in[p];
       const uint32_t mine = (nbp - l + NL - 1u) / NL;  // blocks l, l + NL, ... of p
       for (uint32_t b0 = 0; b0 < mine; b0 += kList) {
@@ -404,7 +425,7 @@ part_round_kernel(PartRound A) {
 #pragma unroll
           for (uint32_t j = 0; j < kPer; ++j)
             if (valid[j]) atomicAdd(&cnt[cur[j] - lo], 1u);
-          if constexpr (kMode == kMid) step_and_scatter(L, A, cur, sec, wid, valid);
+          if constexpr (kMode == kMid) step_and_scatter<true>(L, A, ridx, lo, cur, sec, wid, valid);
 #pragma unroll
           for (uint32_t j = 0; j < kPer; ++j) {
             cur[j] = ncur[j];
@@ -658,7 +679,7 @@ bool part_shape(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint64_t n_src, uint32_t n
   for (uint32_t p = 0; p < S->P; ++p)
     S->vbig = std::max(S->vbig, dg->part_plo[p + 1] - dg->part_plo[p]);
   // every workgroup resident, a multiple of 8 (the XCDs)
-  S->lds = sizeof(PartLds) + static_cast<size_t>(S->vbig) * sizeof(uint32_t);
+  S->lds = sizeof(PartLds) + (2ull * S->vbig + 1) * sizeof(uint32_t);
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
           &per_cu, reinterpret_cast<const void*>(&part_round_kernel<kMid>), kPBlock, S->lds) !=
@@ -669,7 +690,7 @@ bool part_shape(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint64_t n_src, uint32_t n
   S->grid = std::max<uint32_t>(8u, static_cast<uint32_t>(ctx->num_cus * per_cu) / 8u * 8u);
   S->NL = S->grid / 8u;
   // chunk: sources whose walkers fit the record pools
-  const uint64_t budget = static_cast<uint64_t>(std::max(64, env_int("TGTP_WALK_PART_MIB", 24576)))
+  const uint64_t budget = static_cast<uint64_t>(std::max(64, env_int("TGTP_WALK_PART_MIB", 49152)))
                           << 20;
   // a round's regions hold its input blocks plus P per workgroup, and its
   // input holds at most the walkers' full blocks plus P partial blocks per
@@ -701,7 +722,7 @@ int saba_part_prepare(tgtp_ctx* ctx, tgtp_dgraph* dg, uint64_t n_src, uint32_t d
   *use = false;
   if (depth == 0 || !part_env_on(dg)) return TGTP_OK;
   const uint64_t n = dg->n;
-  const uint32_t vmax = static_cast<uint32_t>(std::max(64, env_int("TGTP_WALK_PART_VMAX", 8192)));
+  const uint32_t vmax = static_cast<uint32_t>(std::max(64, env_int("TGTP_WALK_PART_VMAX", 7168)));
   const uint32_t want = static_cast<uint32_t>(std::max(1, env_int("TGTP_WALK_PARTS", 128)));
   const bool required = env_int("TGTP_WALK_PART", -1) > 0;
   auto refuse = [&](const char* why) {
@@ -805,7 +826,9 @@ int launch_saba_walks_part(tgtp_ctx* ctx, tgtp_dgraph* dg, uint32_t src_begin, u
 
   PartRound A{};
   A.rowd = dg->rowd;
+  A.idx = dg->idx;
   A.adj = dg->adj;
+  A.vbig = S.vbig;
   A.plo = static_cast<const uint32_t*>(dg->part_dev);
   A.ptab = reinterpret_cast<const uint16_t*>(static_cast<const uint8_t*>(dg->part_dev) +
                                              (kMaxParts + 1) * sizeof(uint32_t));
