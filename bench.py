@@ -36,6 +36,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "SABA transitions/s + tgtp edges/s at eps 0.01, batched vs plain, 1 GPU"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_TRANSITION = 20  # row bounds 8 + neighbour 4 + counter RMW 8
+RECORD_BYTES_PER_TRANSITION = 24  # partitioned engine: a 12-byte walker record read and written
 
 
 T_START = time.perf_counter()
@@ -102,17 +103,53 @@ def bench_saba(args, g=None, depth=None, steps=None, warmup=None):
                 f"n_walks={n_walks}")
     log(f"[bench] depth {depth}, {steps} steps: {ms_per_step:.2f} ms/step, walk kernel "
         f"{avg_walk_s * 1e3:.2f} ms, {value:.3e} transitions/s, checksum={cs}")
-    # one walk pass = source chunks x (saba_walk_kernel, bucket_count_kernel);
-    # the unit of "launch" here is the pass, timed over all its kernels
+    # The library walks graphs whose rows and adjacency exceed 64 MiB with the
+    # partitioned engine (saba_part.hip: per source chunk one generation
+    # round, depth-1 middle rounds and a counting round of part_round_kernel,
+    # plus the fallback replay) unless TGTP_WALK_PART=0; smaller graphs with
+    # the direct kernel (saba_walk.hip: saba_walk_kernel + bucket_count_kernel
+    # per chunk).  The unit of "launch" here is the pass, timed over all its
+    # kernels with HIP events on the library's stream.
+    part = (os.environ.get("TGTP_WALK_PART", "-1") != "0"
+            and 4 * g.m + 8 * g.n >= 64 * 2**20)
+    if part:
+        kernel = ("part_round_kernel<0,1,2> + part_fixup_kernel + block listing (one pass; "
+                  "the middle rounds part_round_kernel<1> are 90% of it)")
+        traffic = traffic_for("saba_part_pass", f"cl1m_d{depth}_w{n_walks}")
+    else:
+        kernel = "saba_walk_kernel + bucket_count_kernel (one pass)"
+        traffic = traffic_for("saba_walk_pass", f"cl1m_d{depth}_w{n_walks}")
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_for("saba_walk_pass", f"cl1m_d{depth}_w{n_walks}"),
-            "kernel": "saba_walk_kernel + bucket_count_kernel (one pass)",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "engine": "partitioned" if part else "direct", "kernel": kernel,
             "kernel_ms": round(avg_walk_s * 1e3, 3),
             "bytes_per_unit": BYTES_PER_TRANSITION, "units_per_launch": trans}
     cfg = {"workload": workload, "graph": WL.describe(WL.CL1M), "n": g.n, "m": g.m,
            "depth": depth, "n_walks": n_walks, "visit_count": vc, "checksum": cs}
     return value, ms_per_step, roof, cfg, g
+
+
+def stream_copy_rate():
+    """Device-to-device copy rate (bytes read + written per second, GB/s) of
+    a 4 GiB buffer, best of 5, timed with events on the copy's stream."""
+    import torch
+
+    a = torch.empty(1 << 30, dtype=torch.int32, device="cuda:0")
+    a.fill_(1)
+    b = torch.empty_like(a)
+    s = torch.cuda.current_stream()
+    best = None
+    for _ in range(6):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        b.copy_(a)
+        e1.record(s)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * (4 << 30) / (best / 1e3) / 1e9
 
 
 def gather_roof(mib: float):
@@ -702,13 +739,28 @@ def main() -> int:
     value, ms_per_step, roof, cfg, g = bench_saba(args)
     roof_lps = gather_roof(g.m * 8 / 2**20)
     if roof_lps:
-        roof["binding_roof"] = {
+        # the direct kernel's bound: one random cache line per hop
+        roof["random_gather_roof"] = {
             "what": "dependent random 8-byte loads over a buffer of the packed adjacency's "
                     f"size ({g.m * 8 / 2**20:.0f} MiB), best over 1-8 waves/SIMD and 1-4 "
-                    "chains per lane (tgtp_amd/bin/gather_roof, measured in this run)",
+                    "chains per lane (tgtp_amd/bin/gather_roof, measured in this run); "
+                    "the direct kernel runs at 1.1-1.2x this rate",
             "loads_per_s": roof_lps, "transitions_per_s": value, "frac": round(value / roof_lps, 4)}
         log(f"[bench] random-gather roof {roof_lps:.3e} loads/s; walk at "
             f"{value / roof_lps:.2f} of it")
+    copy_gbs = stream_copy_rate()
+    if copy_gbs:
+        # the partitioned engine's bound: each walker's 12-byte record is read
+        # and written once per hop
+        bound = copy_gbs * 1e9 / RECORD_BYTES_PER_TRANSITION
+        roof["record_stream_roof"] = {
+            "what": "a 4 GiB device-to-device copy measured in this run (read + write bytes "
+                    "per second) over the partitioned engine's 24 bytes of records per "
+                    "transition (12 B read, 12 B written)",
+            "copy_GBps": round(copy_gbs, 1), "bytes_per_transition": RECORD_BYTES_PER_TRANSITION,
+            "transitions_per_s": bound, "frac": round(value / bound, 4)}
+        log(f"[bench] record-stream roof {bound:.3e} transitions/s ({copy_gbs:.0f} GB/s copy); "
+            f"walk at {value / bound:.2f} of it")
     line = {"metric": METRIC, "value": value, "unit": "transitions/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
