@@ -110,6 +110,8 @@ struct PartRound {
   uint32_t* reg_used;         // [grid]
   uint32_t* overflow;         // set when a region runs out (never with sized regions)
   uint32_t* copies;           // [grid / 8][n]
+  uint32_t* counters;         // the caller's counters (arrivals of a two-hop generation round)
+  uint32_t gen_hops;          // hops the generation round takes: 1, or 2 when depth >= 2
   uint32_t* taint;            // bouquet bitmap of the chunk
 };
 
@@ -127,6 +129,7 @@ struct PartLds {
   uint32_t rbase;             // first block of this workgroup's region
   uint32_t rsize;             // blocks in it
   uint32_t used;              // blocks taken from it
+  uint32_t hn;                // generation round: entries in the arrival table
   uint32_t plo[kMaxParts + 1];
   uint16_t ptab[kTab + 2];
   uint32_t lblk[kList];       // this workgroup's input blocks of the current partition
@@ -197,13 +200,33 @@ __device__ __forceinline__ void scan_bins(PartLds& L) {
   lds_barrier();
 }
 
+// Counts one arrival at v in the LDS table (open addressing, H a power of two
+// or 0), or directly in the counters when the table has no slot for it.
+__device__ __forceinline__ void count_arrival(PartLds& L, uint32_t* hkey, uint32_t* hval,
+                                              uint32_t* hlist, uint32_t H, uint32_t v,
+                                              uint32_t* counters) {
+  if (H) {
+    uint32_t h = fmix32(v) & (H - 1u);
+    for (int probe = 0; probe < 8; ++probe) {
+      const uint32_t k = atomicCAS(&hkey[h], kNone, v);
+      if (k == kNone || k == v) {
+        atomicAdd(&hval[h], 1u);
+        if (k == kNone) hlist[atomicAdd(&L.hn, 1u)] = h;
+        return;
+      }
+      h = (h + 1u) & (H - 1u);
+    }
+  }
+  atomicAdd(&counters[v], 1u);
+}
+
 // One tile: each thread holds up to kPer walkers (valid[j]) at vertex cur[j]
 // that take hop `round`.  Sorts the successors by partition and appends them
 // to the workgroup's blocks.  kLdsRows: the walkers are in the partition whose
 // row offsets ridx[0, V] (vertices lo + i) are staged in LDS, so the hop's only
 // gather is the neighbour itself; otherwise the row descriptors are gathered.
 template <bool kLdsRows>
-__device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A,
+__device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A, uint32_t hop,
                                                  const uint32_t* ridx, uint32_t lo,
                                                  uint32_t (&cur)[kPer], uint32_t (&sec)[kPer],
                                                  uint32_t (&wid)[kPer], bool (&valid)[kPer]) {
@@ -227,7 +250,7 @@ __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A,
     if (valid[j]) {
       k = __umulhi(sec[j] ^ fmix32(cur[j]), rd[j].y);
       const uint32_t ns = sec[j] * rd[j].y;
-      if (ns == 0u && sec[j] != 0u && A.round + 1u < A.depth) mark_bouquet(A.taint, wid[j]);
+      if (ns == 0u && sec[j] != 0u && hop + 1u < A.depth) mark_bouquet(A.taint, wid[j]);
       sec[j] = ns;
     }
     nxt[j] = valid[j] ? A.adj[rd[j].x + k] : 0u;
@@ -376,6 +399,24 @@ part_round_kernel(PartRound A) {
   }
 
   if constexpr (kMode == kGen) {
+    // arrival table of a two-hop generation round, in the space the other
+    // rounds use for counters and row offsets: H slots of key, count and
+    // the list of slots in use
+    uint32_t H = 0;
+    if (A.gen_hops == 2u) {
+      const uint32_t words = 2u * A.vbig + 1u;
+      while ((H ? H << 1 : 1u) * 3u <= words && H < (1u << 16)) H = H ? H << 1 : 1u;
+      if (H < 64u) H = 0;  // too small to help: every arrival goes to the counters
+    }
+    uint32_t* hkey = cnt;
+    uint32_t* hval = cnt + H;
+    uint32_t* hlist = cnt + 2u * H;
+    for (uint32_t i = tid; i < H; i += kPBlock) {
+      hkey[i] = kNone;
+      hval[i] = 0u;
+    }
+    if (tid == 0) L.hn = 0u;
+    lds_barrier();
     const uint32_t n_tiles = (A.n_walkers + kBR - 1u) / kBR;
     for (uint32_t t = w; t < n_tiles; t += gridDim.x) {
       uint32_t cur[kPer], sec[kPer], wid[kPer];
@@ -390,7 +431,41 @@ part_round_kernel(PartRound A) {
         sec[j] = (i + 1u) * A.step;
         if (valid[j] && sec[j] == 0u) mark_bouquet(A.taint, wid[j]);
       }
-      step_and_scatter<false>(L, A, nullptr, 0u, cur, sec, wid, valid);
+      if (A.gen_hops == 2u) {
+        // first hop here: a tile's walkers share one or two sources, so their
+        // successors are a few vertices whose rows the second hop (in
+        // step_and_scatter) reads from cache; the arrivals of the first hop
+        // are aggregated in the LDS table and added to the caller's counters
+        uint2 rd[kPer];
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j) rd[j] = valid[j] ? A.rowd[cur[j]] : make_uint2(0u, 0u);
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j) {
+          valid[j] = valid[j] && rd[j].y != 0u;
+          if (valid[j]) {
+            const uint32_t k = __umulhi(sec[j] ^ fmix32(cur[j]), rd[j].y);
+            const uint32_t ns = sec[j] * rd[j].y;
+            if (ns == 0u && sec[j] != 0u) mark_bouquet(A.taint, wid[j]);  // depth >= 2
+            sec[j] = ns;
+            cur[j] = A.adj[rd[j].x + k];
+            count_arrival(L, hkey, hval, hlist, H, cur[j], A.counters);
+          }
+        }
+        step_and_scatter<false>(L, A, 1u, nullptr, 0u, cur, sec, wid, valid);
+        // the barrier at the end of step_and_scatter follows every insert
+        const uint32_t nh = L.hn;
+        for (uint32_t i = tid; i < nh; i += kPBlock) {
+          const uint32_t h = hlist[i];
+          atomicAdd(&A.counters[hkey[h]], hval[h]);
+          hkey[h] = kNone;
+          hval[h] = 0u;
+        }
+        lds_barrier();
+        if (tid == 0) L.hn = 0u;
+        lds_barrier();
+      } else {
+        step_and_scatter<false>(L, A, 0u, nullptr, 0u, cur, sec, wid, valid);
+      }
     }
   } else {
     uint32_t* copy = A.copies + static_cast<uint64_t>(l) * A.n;
@@ -425,7 +500,8 @@ part_round_kernel(PartRound A) {
 #pragma unroll
           for (uint32_t j = 0; j < kPer; ++j)
             if (valid[j]) atomicAdd(&cnt[cur[j] - lo], 1u);
-          if constexpr (kMode == kMid) step_and_scatter<true>(L, A, ridx, lo, cur, sec, wid, valid);
+          if constexpr (kMode == kMid)
+            step_and_scatter<true>(L, A, A.round, ridx, lo, cur, sec, wid, valid);
 #pragma unroll
           for (uint32_t j = 0; j < kPer; ++j) {
             cur[j] = ncur[j];
@@ -840,6 +916,8 @@ int launch_saba_walks_part(tgtp_ctx* ctx, tgtp_dgraph* dg, uint32_t src_begin, u
   A.step = 0xFFFFFFFFu / n_walks;
   A.src_stride = static_cast<uint32_t>(n_chunks);
   A.pool_cap = pool_cap;
+  A.counters = d_counters;
+  A.gen_hops = depth >= 2 && env_int("TGTP_WALK_PART_GEN2", 1) != 0 ? 2u : 1u;
   A.cwd = cwd;
   A.reg_start = reg_start;
   A.reg_used = reg_used;
@@ -866,6 +944,7 @@ int launch_saba_walks_part(tgtp_ctx* ctx, tgtp_dgraph* dg, uint32_t src_begin, u
     fill(taint, words);
     int cur = 0;
     for (uint32_t r = 0; r <= depth; ++r) {
+      if (r == 1 && A.gen_hops == 2u) continue;  // taken by the generation round
       const Pool& in = pool[cur ^ 1];
       const Pool& out = pool[cur];
       A.round = r;
