@@ -80,7 +80,8 @@ def test_walker_stops_at_vertex_without_neighbours(gpu_ctx, part):
     {"TGTP_WALK_PART_PER_CU": "1"},                     # one workgroup per CU
     {"TGTP_WALK_PARTS": "1", "TGTP_WALK_PART_VMAX": "8192"},  # a single partition
     {"TGTP_WALK_PARTS": "256", "TGTP_WALK_PART_VMAX": "16"},  # the most sort bins
-    {"TGTP_WALK_PART_GEN2": "0"},                       # one hop in the generation round
+    {"TGTP_WALK_PART_GEN_HOPS": "1"},                   # one hop in the generation round
+    {"TGTP_WALK_PART_GEN_HOPS": "4"},                   # four
     {"TGTP_WALK_PART_VMAX": "512"},                     # arrival table of 256 slots
 ])
 def test_engine_shapes(gpu_ctx, part, knobs):

@@ -111,7 +111,7 @@ struct PartRound {
   uint32_t* overflow;         // set when a region runs out (never with sized regions)
   uint32_t* copies;           // [grid / 8][n]
   uint32_t* counters;         // the caller's counters (arrivals of a two-hop generation round)
-  uint32_t gen_hops;          // hops the generation round takes: 1, or 2 when depth >= 2
+  uint32_t gen_hops;          // hops the generation round takes (at most depth)
   uint32_t* taint;            // bouquet bitmap of the chunk
 };
 
@@ -403,7 +403,7 @@ part_round_kernel(PartRound A) {
     // rounds use for counters and row offsets: H slots of key, count and
     // the list of slots in use
     uint32_t H = 0;
-    if (A.gen_hops == 2u) {
+    if (A.gen_hops > 1u) {
       const uint32_t words = 2u * A.vbig + 1u;
       while ((H ? H << 1 : 1u) * 3u <= words && H < (1u << 16)) H = H ? H << 1 : 1u;
       if (H < 64u) H = 0;  // too small to help: every arrival goes to the counters
@@ -431,27 +431,30 @@ part_round_kernel(PartRound A) {
         sec[j] = (i + 1u) * A.step;
         if (valid[j] && sec[j] == 0u) mark_bouquet(A.taint, wid[j]);
       }
-      if (A.gen_hops == 2u) {
-        // first hop here: a tile's walkers share one or two sources, so their
-        // successors are a few vertices whose rows the second hop (in
-        // step_and_scatter) reads from cache; the arrivals of the first hop
-        // are aggregated in the LDS table and added to the caller's counters
-        uint2 rd[kPer];
+      if (A.gen_hops > 1u) {
+        // the first hops here: a tile's walkers share one or two sources, so
+        // their successors are a few vertices (a few hundred after two hops)
+        // whose rows stay in cache; the arrivals of these hops are aggregated
+        // in the LDS table and added to the caller's counters, the last hop
+        // is step_and_scatter's
+        for (uint32_t hop = 0; hop + 1u < A.gen_hops; ++hop) {
+          uint2 rd[kPer];
 #pragma unroll
-        for (uint32_t j = 0; j < kPer; ++j) rd[j] = valid[j] ? A.rowd[cur[j]] : make_uint2(0u, 0u);
+          for (uint32_t j = 0; j < kPer; ++j) rd[j] = valid[j] ? A.rowd[cur[j]] : make_uint2(0u, 0u);
 #pragma unroll
-        for (uint32_t j = 0; j < kPer; ++j) {
-          valid[j] = valid[j] && rd[j].y != 0u;
-          if (valid[j]) {
-            const uint32_t k = __umulhi(sec[j] ^ fmix32(cur[j]), rd[j].y);
-            const uint32_t ns = sec[j] * rd[j].y;
-            if (ns == 0u && sec[j] != 0u) mark_bouquet(A.taint, wid[j]);  // depth >= 2
-            sec[j] = ns;
-            cur[j] = A.adj[rd[j].x + k];
-            count_arrival(L, hkey, hval, hlist, H, cur[j], A.counters);
+          for (uint32_t j = 0; j < kPer; ++j) {
+            valid[j] = valid[j] && rd[j].y != 0u;
+            if (valid[j]) {
+              const uint32_t k = __umulhi(sec[j] ^ fmix32(cur[j]), rd[j].y);
+              const uint32_t ns = sec[j] * rd[j].y;
+              if (ns == 0u && sec[j] != 0u) mark_bouquet(A.taint, wid[j]);  // hop + 1 < depth
+              sec[j] = ns;
+              cur[j] = A.adj[rd[j].x + k];
+              count_arrival(L, hkey, hval, hlist, H, cur[j], A.counters);
+            }
           }
         }
-        step_and_scatter<false>(L, A, 1u, nullptr, 0u, cur, sec, wid, valid);
+        step_and_scatter<false>(L, A, A.gen_hops - 1u, nullptr, 0u, cur, sec, wid, valid);
         // the barrier at the end of step_and_scatter follows every insert
         const uint32_t nh = L.hn;
         for (uint32_t i = tid; i < nh; i += kPBlock) {
@@ -917,7 +920,8 @@ int launch_saba_walks_part(tgtp_ctx* ctx, tgtp_dgraph* dg, uint32_t src_begin, u
   A.src_stride = static_cast<uint32_t>(n_chunks);
   A.pool_cap = pool_cap;
   A.counters = d_counters;
-  A.gen_hops = depth >= 2 && env_int("TGTP_WALK_PART_GEN2", 1) != 0 ? 2u : 1u;
+  A.gen_hops = std::min<uint32_t>(depth, static_cast<uint32_t>(
+                                           std::max(1, env_int("TGTP_WALK_PART_GEN_HOPS", 2))));
   A.cwd = cwd;
   A.reg_start = reg_start;
   A.reg_used = reg_used;
@@ -944,7 +948,7 @@ int launch_saba_walks_part(tgtp_ctx* ctx, tgtp_dgraph* dg, uint32_t src_begin, u
     fill(taint, words);
     int cur = 0;
     for (uint32_t r = 0; r <= depth; ++r) {
-      if (r == 1 && A.gen_hops == 2u) continue;  // taken by the generation round
+      if (r > 0 && r < A.gen_hops) continue;  // taken by the generation round
       const Pool& in = pool[cur ^ 1];
       const Pool& out = pool[cur];
       A.round = r;
