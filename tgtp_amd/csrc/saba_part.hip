@@ -225,11 +225,38 @@ __device__ __forceinline__ void count_arrival(PartLds& L, uint32_t* hkey, uint32
 // to the workgroup's blocks.  kLdsRows: the walkers are in the partition whose
 // row offsets ridx[0, V] (vertices lo + i) are staged in LDS, so the hop's only
 // gather is the neighbour itself; otherwise the row descriptors are gathered.
+// Writes the sorted tile held in LDS to its blocks.
+__device__ __forceinline__ void copy_out(PartLds& L, const PartRound& A) {
+  const uint32_t total = L.total;
+  for (uint32_t q = threadIdx.x; q < total; q += kPBlock) {
+    const uint32_t d = L.tdest[q];
+    const uint32_t jj = q - L.off[d];
+    const uint32_t l0 = L.len0[d];
+    uint32_t g;
+    if (jj < l0) {
+      g = L.seg0[d] + jj;
+    } else {
+      const uint32_t s1 = L.seg1[d];
+      if (s1 == kNone) continue;  // region overflow: reported, records dropped
+      g = s1 + (jj - l0);
+    }
+    uint32_t* r = A.rec_out + static_cast<uint64_t>(g) * 3u;
+    __builtin_nontemporal_store(L.tcur[q], r);
+    __builtin_nontemporal_store(L.tsec[q], r + 1);
+    __builtin_nontemporal_store(L.twid[q], r + 2);
+  }
+}
+
+// `pending`: LDS holds the previous tile, sorted; its records are written
+// after this tile's neighbour gathers are issued.  Vector-memory completions
+// are counted in issue order, so stores issued before the gathers would hold
+// up the wait for them.  The caller writes the last tile (copy_out).
 template <bool kLdsRows>
 __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A, uint32_t hop,
                                                  const uint32_t* ridx, uint32_t lo,
                                                  uint32_t (&cur)[kPer], uint32_t (&sec)[kPer],
-                                                 uint32_t (&wid)[kPer], bool (&valid)[kPer]) {
+                                                 uint32_t (&wid)[kPer], bool (&valid)[kPer],
+                                                 bool& pending) {
   const uint32_t tid = threadIdx.x;
   uint2 rd[kPer];
 #pragma unroll
@@ -255,6 +282,7 @@ __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A,
     }
     nxt[j] = valid[j] ? A.adj[rd[j].x + k] : 0u;
   }
+  if (pending) copy_out(L, A);
 #pragma unroll
   for (uint32_t j = 0; j < kPer; ++j) {
     dest[j] = 0u;
@@ -294,6 +322,7 @@ __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A,
     L.curblk[tid] = cb;
     L.curfill[tid] = cf;
   }
+  if (tid < kMaxParts) L.hist[tid] = 0u;  // read above by this thread only
 #pragma unroll
   for (uint32_t j = 0; j < kPer; ++j) {
     if (valid[j]) {
@@ -305,26 +334,7 @@ __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A,
     }
   }
   lds_barrier();
-  const uint32_t total = L.total;
-  for (uint32_t q = tid; q < total; q += kPBlock) {
-    const uint32_t d = L.tdest[q];
-    const uint32_t jj = q - L.off[d];
-    const uint32_t l0 = L.len0[d];
-    uint32_t g;
-    if (jj < l0) {
-      g = L.seg0[d] + jj;
-    } else {
-      const uint32_t s1 = L.seg1[d];
-      if (s1 == kNone) continue;  // region overflow: reported, records dropped
-      g = s1 + (jj - l0);
-    }
-    uint32_t* r = A.rec_out + static_cast<uint64_t>(g) * 3u;
-    __builtin_nontemporal_store(L.tcur[q], r);
-    __builtin_nontemporal_store(L.tsec[q], r + 1);
-    __builtin_nontemporal_store(L.twid[q], r + 2);
-  }
-  if (tid < kMaxParts) L.hist[tid] = 0u;
-  lds_barrier();
+  pending = true;
 }
 
 // Loads input record i of block blk (kNone: none) as vertex / secret / id.
@@ -398,6 +408,7 @@ part_round_kernel(PartRound A) {
     lds_barrier();
   }
 
+  bool pending = false;  // LDS holds a sorted tile not yet written
   if constexpr (kMode == kGen) {
     // arrival table of a two-hop generation round, in the space the other
     // rounds use for counters and row offsets: H slots of key, count and
@@ -454,7 +465,7 @@ part_round_kernel(PartRound A) {
             }
           }
         }
-        step_and_scatter<false>(L, A, A.gen_hops - 1u, nullptr, 0u, cur, sec, wid, valid);
+        step_and_scatter<false>(L, A, A.gen_hops - 1u, nullptr, 0u, cur, sec, wid, valid, pending);
         // the barrier at the end of step_and_scatter follows every insert
         const uint32_t nh = L.hn;
         for (uint32_t i = tid; i < nh; i += kPBlock) {
@@ -467,7 +478,7 @@ part_round_kernel(PartRound A) {
         if (tid == 0) L.hn = 0u;
         lds_barrier();
       } else {
-        step_and_scatter<false>(L, A, 0u, nullptr, 0u, cur, sec, wid, valid);
+        step_and_scatter<false>(L, A, 0u, nullptr, 0u, cur, sec, wid, valid, pending);
       }
     }
   } else {
@@ -504,7 +515,7 @@ part_round_kernel(PartRound A) {
           for (uint32_t j = 0; j < kPer; ++j)
             if (valid[j]) atomicAdd(&cnt[cur[j] - lo], 1u);
           if constexpr (kMode == kMid)
-            step_and_scatter<true>(L, A, A.round, ridx, lo, cur, sec, wid, valid);
+            step_and_scatter<true>(L, A, A.round, ridx, lo, cur, sec, wid, valid, pending);
 #pragma unroll
           for (uint32_t j = 0; j < kPer; ++j) {
             cur[j] = ncur[j];
@@ -521,6 +532,7 @@ part_round_kernel(PartRound A) {
       }
     }
   }
+  if (pending) copy_out(L, A);
   if constexpr (kMode != kLast) {
     lds_barrier();
     if (tid < A.P) {
