@@ -90,6 +90,17 @@ def test_engine_shapes(gpu_ctx, part, knobs):
     _check(gpu_ctx, G.chung_lu(3000, 2.3, 14.0, seed=5), 9, 64, label=f"part {knobs}")
 
 
+@pytest.mark.parametrize("W_,slots", [(6144, "1"), (12288, "1"), (16384, "1"), (16384, "0")])
+def test_generation_first_hop_counting(gpu_ctx, part, W_, slots):
+    """Two-hop generation round with walks per source of at least a tile
+    (6,144): first-hop arrivals counted per neighbour slot of the tile's one
+    or two sources (slots "1"), or in the LDS table ("0")."""
+    part.setenv("TGTP_WALK_PARTS", "4")
+    part.setenv("TGTP_WALK_PART_VMAX", "4096")
+    part.setenv("TGTP_WALK_PART_GEN_SLOTS", slots)
+    _check(gpu_ctx, G.chung_lu(3000, 2.3, 14.0, seed=5), 6, W_, 0, 700, label=f"part gen W{W_}")
+
+
 def test_counts_beyond_one_block_per_partition(gpu_ctx, part):
     """Many blocks per partition and per workgroup: 70,000 vertices, 128
     partitions of at most 1024 vertices, 2^27 walks over 4 chunks."""

@@ -112,6 +112,8 @@ struct PartRound {
   uint32_t* copies;           // [grid / 8][n]
   uint32_t* counters;         // the caller's counters (arrivals of a two-hop generation round)
   uint32_t gen_hops;          // hops the generation round takes (at most depth)
+  uint32_t gen_slots;         // two-hop generation, W >= tile: first-hop arrivals counted
+                              // per neighbour slot of the tile's (at most two) sources
   uint32_t* taint;            // bouquet bitmap of the chunk
 };
 
@@ -422,9 +424,13 @@ part_round_kernel(PartRound A) {
     uint32_t* hkey = cnt;
     uint32_t* hval = cnt + H;
     uint32_t* hlist = cnt + 2u * H;
-    for (uint32_t i = tid; i < H; i += kPBlock) {
-      hkey[i] = kNone;
-      hval[i] = 0u;
+    if (A.gen_slots) {
+      for (uint32_t i = tid; i < 2u * A.vbig + 1u; i += kPBlock) cnt[i] = 0u;
+    } else {
+      for (uint32_t i = tid; i < H; i += kPBlock) {
+        hkey[i] = kNone;
+        hval[i] = 0u;
+      }
     }
     if (tid == 0) L.hn = 0u;
     lds_barrier();
@@ -442,7 +448,42 @@ part_round_kernel(PartRound A) {
         sec[j] = (i + 1u) * A.step;
         if (valid[j] && sec[j] == 0u) mark_bouquet(A.taint, wid[j]);
       }
-      if (A.gen_hops > 1u) {
+      if (A.gen_slots) {
+        // the first hop here, arrivals counted per neighbour slot of the
+        // tile's sources (uniform: s0 holds wids from the tile's first on, s1
+        // the rest); the second hop reads the rows of those neighbours from
+        // cache in step_and_scatter
+        const uint32_t w0 = t * kBR, w1 = min(w0 + kBR, A.n_walkers) - 1u;
+        const uint32_t sl0 = w0 / A.W, sl1 = w1 / A.W;
+        const uint2 r0 = A.rowd[A.src_first + sl0 * A.src_stride];
+        const uint2 r1 = A.rowd[A.src_first + sl1 * A.src_stride];
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j) {
+          const bool second = valid[j] && wid[j] / A.W != sl0;
+          const uint2 rd = second ? r1 : r0;
+          valid[j] = valid[j] && rd.y != 0u;
+          if (valid[j]) {
+            const uint32_t k = __umulhi(sec[j] ^ fmix32(cur[j]), rd.y);
+            const uint32_t ns = sec[j] * rd.y;
+            if (ns == 0u && sec[j] != 0u) mark_bouquet(A.taint, wid[j]);  // depth >= 2
+            sec[j] = ns;
+            cur[j] = A.adj[rd.x + k];
+            atomicAdd(&cnt[(second ? r0.y : 0u) + k], 1u);
+          }
+        }
+        step_and_scatter<false>(L, A, 1u, nullptr, 0u, cur, sec, wid, valid, pending);
+        // the barrier at the end of step_and_scatter follows every count
+        const uint32_t ns0 = r0.y, nsl = r0.y + (sl1 != sl0 ? r1.y : 0u);
+        for (uint32_t i = tid; i < nsl; i += kPBlock) {
+          const uint32_t c = cnt[i];
+          if (c) {
+            const uint32_t v = i < ns0 ? A.adj[r0.x + i] : A.adj[r1.x + (i - ns0)];
+            atomicAdd(&A.counters[v], c);
+            cnt[i] = 0u;
+          }
+        }
+        lds_barrier();
+      } else if (A.gen_hops > 1u) {
         // the first hops here: a tile's walkers share one or two sources, so
         // their successors are a few vertices (a few hundred after two hops)
         // whose rows stay in cache; the arrivals of these hops are aggregated
@@ -934,6 +975,8 @@ int launch_saba_walks_part(tgtp_ctx* ctx, tgtp_dgraph* dg, uint32_t src_begin, u
   A.counters = d_counters;
   A.gen_hops = std::min<uint32_t>(depth, static_cast<uint32_t>(
                                            std::max(1, env_int("TGTP_WALK_PART_GEN_HOPS", 2))));
+  A.gen_slots = A.gen_hops == 2u && n_walks >= kBR && 2ull * dg->max_deg <= 2ull * S.vbig + 1 &&
+                env_int("TGTP_WALK_PART_GEN_SLOTS", 1) != 0;
   A.cwd = cwd;
   A.reg_start = reg_start;
   A.reg_used = reg_used;
