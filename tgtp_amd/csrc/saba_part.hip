@@ -693,61 +693,78 @@ __device__ __forceinline__ void hop_independent(const FixParams& F, uint32_t& cu
   cur = F.adj[rd.x + k];
 }
 
-// Replays every marked bouquet: t* = first hop at whose start a live lane
-// holds sec == 0; from t* on each lane's independent visits are subtracted
-// and its SplitMix32 visits added.
+// Replays every marked bouquet, one 8-lane group per bouquet, a lane per
+// walker: t* = first hop at whose start a live lane holds sec == 0 (a
+// minimum over the group); from t* on each lane's independent visits are
+// subtracted and its SplitMix32 visits added.  A wave scans 64 bitmap words at
+// a time and hands their marked bouquets to its eight groups.
 __global__ void __launch_bounds__(256)
 part_fixup_kernel(FixParams F) {
-  for (uint32_t w = blockIdx.x * 256u + threadIdx.x; w < F.n_words; w += gridDim.x * 256u) {
-    uint32_t bits = F.taint[w];
-    while (bits) {
-      const uint32_t bq = w * 32u + static_cast<uint32_t>(__builtin_ctz(bits));
-      bits &= bits - 1u;
-      const uint32_t wid0 = bq * 8u;
-      const uint32_t sl = wid0 / F.W;
-      const uint32_t i0 = wid0 - sl * F.W;
-      const uint32_t src = F.src_first + sl * F.src_stride;
-      if (F.rowd[src].y == 0u) continue;
-      uint32_t ts = F.depth;
-      for (uint32_t b = 0; b < 8u; ++b) {
-        uint32_t cur = src, sec = (i0 + b + 1u) * F.step;
-        for (uint32_t t = 0; t < F.depth && t < ts; ++t) {
-          const uint2 rd = F.rowd[cur];
-          if (rd.y == 0u) break;
-          if (sec == 0u) {
-            ts = t;
-            break;
+  const uint32_t lane = threadIdx.x & 63u, grp = lane >> 3, b = lane & 7u;
+  const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6;
+  const uint32_t n_waves = (gridDim.x * 256u) >> 6;
+  for (uint32_t base = wave * 64u; base < F.n_words; base += n_waves * 64u) {
+    const uint32_t mine = base + lane < F.n_words ? F.taint[base + lane] : 0u;
+    uint64_t nz = __ballot(mine != 0u);
+    while (nz) {  // uniform per wave
+      const uint32_t L0 = static_cast<uint32_t>(__builtin_ctzll(nz));
+      nz &= nz - 1ull;
+      uint32_t bits = __shfl(mine, static_cast<int>(L0));
+      const uint32_t word = base + L0;
+      while (bits) {
+        // group g takes the g-th lowest marked bouquet of what is left
+        uint32_t left = bits;
+        for (uint32_t g = 0; g < grp && left; ++g) left &= left - 1u;
+        const bool has = left != 0u;
+        const uint32_t bq = has ? word * 32u + static_cast<uint32_t>(__builtin_ctz(left)) : 0u;
+        for (uint32_t g = 0; g < 8u && bits; ++g) bits &= bits - 1u;
+        const uint32_t wid0 = bq * 8u;
+        const uint32_t sl = wid0 / F.W;
+        const uint32_t init = (wid0 - sl * F.W + b + 1u) * F.step;
+        const uint32_t src = F.src_first + sl * F.src_stride;
+        const bool go = has && F.rowd[src].y != 0u;
+        uint32_t tb = F.depth;  // this lane's first hop with a zero secret while live
+        if (go) {
+          uint32_t cur = src, sec = init;
+          for (uint32_t t = 0; t < F.depth; ++t) {
+            const uint2 rd = F.rowd[cur];
+            if (rd.y == 0u) break;
+            if (sec == 0u) {
+              tb = t;
+              break;
+            }
+            hop_independent(F, cur, sec, rd);
           }
-          hop_independent(F, cur, sec, rd);
         }
-      }
-      if (ts >= F.depth) continue;
-      for (uint32_t b = 0; b < 8u; ++b) {
-        const uint32_t init = (i0 + b + 1u) * F.step;
-        uint32_t cur = src, sec = init;
-        bool live = true;
-        for (uint32_t t = 0; t < ts; ++t) {
-          const uint2 rd = F.rowd[cur];
-          if (rd.y == 0u) {
-            live = false;
-            break;
+        uint32_t ts = tb;
+        for (int o = 1; o < 8; o <<= 1) ts = min(ts, static_cast<uint32_t>(__shfl_xor(ts, o)));
+        if (go && ts < F.depth) {
+          uint32_t cur = src, sec = init;
+          bool live = true;
+          for (uint32_t t = 0; t < ts; ++t) {
+            const uint2 rd = F.rowd[cur];
+            if (rd.y == 0u) {
+              live = false;
+              break;
+            }
+            hop_independent(F, cur, sec, rd);
           }
-          hop_independent(F, cur, sec, rd);
-        }
-        if (!live) continue;
-        uint32_t c1 = cur, s1 = sec;
-        for (uint32_t t = ts; t < F.depth; ++t) {
-          const uint2 rd = F.rowd[c1];
-          if (rd.y == 0u) break;
-          hop_independent(F, c1, s1, rd);
-          atomicAdd(&F.counters[c1], 0xFFFFFFFFu);
-        }
-        uint32_t c2 = cur, st = init;
-        for (uint32_t t = ts; t < F.depth; ++t) {
-          const uint2 rd = F.rowd[c2];
-          if (rd.y == 0u) break;
-          c2 = F.adj[rd.x + splitmix32(st) % rd.y];
-          atomicAdd(&F.counters[c2], 1u);
+          if (live) {
+            uint32_t c1 = cur, s1 = sec;
+            for (uint32_t t = ts; t < F.depth; ++t) {
+              const uint2 rd = F.rowd[c1];
+              if (rd.y == 0u) break;
+              hop_independent(F, c1, s1, rd);
+              atomicAdd(&F.counters[c1], 0xFFFFFFFFu);
+            }
+            uint32_t c2 = cur, st = init;
+            for (uint32_t t = ts; t < F.depth; ++t) {
+              const uint2 rd = F.rowd[c2];
+              if (rd.y == 0u) break;
+              c2 = F.adj[rd.x + splitmix32(st) % rd.y];
+              atomicAdd(&F.counters[c2], 1u);
+            }
+          }
         }
       }
     }
@@ -1035,7 +1052,7 @@ int launch_saba_walks_part(tgtp_ctx* ctx, tgtp_dgraph* dg, uint32_t src_begin, u
     }
     F.src_first = A.src_first;
     F.n_words = static_cast<uint32_t>(words);
-    part_fixup_kernel<<<static_cast<uint32_t>(std::min<uint64_t>(4096, (words + 255) / 256)), 256,
+    part_fixup_kernel<<<static_cast<uint32_t>(std::min<uint64_t>(2048, (words + 255) / 256)), 256,
                         0, s>>>(F);
     TGTP_HIP_CHECK(ctx, hipGetLastError());
   }
