@@ -109,6 +109,16 @@ def test_counts_beyond_one_block_per_partition(gpu_ctx, part):
     _check(gpu_ctx, G.chung_lu(70000, 2.3, 12.0, seed=9), 7, 1920, label="part cl70000")
 
 
+def test_hub_arrivals_beyond_16_bits(gpu_ctx, part):
+    """Star of 2,001 vertices, 16,384 walks from every vertex: half of each
+    round's 32.8M walkers stand on the hub, about 256,000 per workgroup copy,
+    so the hub's 16-bit LDS counter is added into the copy and cleared many
+    times within one partition (at most 65,535 arrivals between two)."""
+    g = G.Graph.from_edges(2001, [(0, i) for i in range(1, 2001)])
+    cnt, _ = _check(gpu_ctx, g, 5, 16384, label="part star2001")
+    assert cnt[0] > 4 * 65535 * 64
+
+
 def test_required_engine_refuses_too_many_ranges(gpu_ctx, part):
     part.setenv("TGTP_WALK_PART_VMAX", "64")
     g = G.chung_lu(50000, 2.2, 15.0, seed=4)  # > 256 ranges of 64 vertices

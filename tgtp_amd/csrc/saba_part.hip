@@ -59,12 +59,24 @@ int env_int(const char* name, int dflt) {
 #ifndef TGTP_PART_PER
 #define TGTP_PART_PER 6
 #endif
+#ifndef TGTP_PART_CNT16
+#define TGTP_PART_CNT16 1
+#endif
 constexpr int kPBlock = 1024;                // threads per workgroup
 constexpr uint32_t kPer = TGTP_PART_PER;     // records per thread per tile
 constexpr uint32_t kBR = kPBlock * kPer;     // records per block = per tile
 constexpr uint32_t kMaxParts = 256;          // partitions (= sort bins) at most
 constexpr uint32_t kTab = 2048;              // coarse vertex -> partition table
 constexpr uint32_t kList = 512;              // block ids of a partition staged in LDS at a time
+// Arrival counters of the partition a workgroup serves, in LDS: 16 bits per
+// vertex (two to a word, added into the workgroup's copy every kFlushTiles
+// tiles, which is at most 65,535 arrivals, so no half carries into the other)
+// or 32.  At 2 bytes a vertex, partitions of 10,240 vertices fit beside the
+// tile buffers in 160 KiB of LDS (8,192 at 4 bytes): fewer partitions, fewer
+// partial blocks per round.
+constexpr bool kCnt16 = TGTP_PART_CNT16 != 0;
+constexpr uint32_t kFlushTiles = 65535u / kBR;
+static_assert(kFlushTiles >= 1, "a tile's arrivals fit 16 bits");
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint32_t kMaxPoolBlocks = 0xFFFFFFFFu / kBR;  // record index fits 32 bits
 
@@ -222,7 +234,7 @@ __device__ __forceinline__ void count_arrival(PartLds& L, uint32_t* hkey, uint32
   atomicAdd(&counters[v], 1u);
 }
 
-// One tile: each thread holds up to kPer walkers (valid[j]) at vertex cur[j]
+// One tile: each thread holds up to kPer walkers at vertex cur[j] (kNone: none)
 // that take hop `round`.  Sorts the successors by partition and appends them
 // to the workgroup's blocks.  kLdsRows: the walkers are in the partition whose
 // row offsets ridx[0, V] (vertices lo + i) are staged in LDS, so the hop's only
@@ -257,9 +269,11 @@ template <bool kLdsRows>
 __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A, uint32_t hop,
                                                  const uint32_t* ridx, uint32_t lo,
                                                  uint32_t (&cur)[kPer], uint32_t (&sec)[kPer],
-                                                 uint32_t (&wid)[kPer], bool (&valid)[kPer],
-                                                 bool& pending) {
+                                                 uint32_t (&wid)[kPer], bool& pending) {
   const uint32_t tid = threadIdx.x;
+  bool valid[kPer];  // kNone marks an empty slot
+#pragma unroll
+  for (uint32_t j = 0; j < kPer; ++j) valid[j] = cur[j] != kNone;
   uint2 rd[kPer];
 #pragma unroll
   for (uint32_t j = 0; j < kPer; ++j) {
@@ -339,16 +353,17 @@ __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A,
   pending = true;
 }
 
-// Loads input record i of block blk (kNone: none) as vertex / secret / id.
+// Loads input record i of block blk (kNone: none) as vertex / secret / id;
+// slots past the block's fill get vertex kNone.
 __device__ __forceinline__ void load_tile(const PartRound& A, uint32_t blk, uint32_t fill,
                                           uint32_t (&cur)[kPer], uint32_t (&sec)[kPer],
-                                          uint32_t (&wid)[kPer], bool (&valid)[kPer]) {
+                                          uint32_t (&wid)[kPer]) {
 #pragma unroll
   for (uint32_t j = 0; j < kPer; ++j) {
     const uint32_t i = j * kPBlock + threadIdx.x;
-    valid[j] = blk != kNone && i < fill;
-    cur[j] = sec[j] = wid[j] = 0u;
-    if (valid[j]) {
+    cur[j] = kNone;
+    sec[j] = wid[j] = 0u;
+    if (blk != kNone && i < fill) {
       const uint32_t* r = A.rec_in + (static_cast<uint64_t>(blk) * kBR + i) * 3u;
       cur[j] = __builtin_nontemporal_load(r);
       sec[j] = __builtin_nontemporal_load(r + 1);
@@ -364,7 +379,9 @@ part_round_kernel(PartRound A) {
   PartLds& L = *reinterpret_cast<PartLds*>(lds_raw);
   // the current partition's arrival counters and row offsets
   uint32_t* cnt = reinterpret_cast<uint32_t*>(lds_raw + sizeof(PartLds));
-  uint32_t* ridx = cnt + A.vbig;
+  const uint32_t cnt_words = kCnt16 ? (A.vbig + 1u) / 2u : A.vbig;
+  uint32_t* ridx = cnt + cnt_words;
+  const uint32_t area = cnt_words + A.vbig + 1u;  // words after PartLds
   const uint32_t tid = threadIdx.x;
   const uint32_t w = blockIdx.x;
   const uint32_t xc = w & 7u;
@@ -417,7 +434,7 @@ part_round_kernel(PartRound A) {
     // the list of slots in use
     uint32_t H = 0;
     if (A.gen_hops > 1u) {
-      const uint32_t words = 2u * A.vbig + 1u;
+      const uint32_t words = area;
       while ((H ? H << 1 : 1u) * 3u <= words && H < (1u << 16)) H = H ? H << 1 : 1u;
       if (H < 64u) H = 0;  // too small to help: every arrival goes to the counters
     }
@@ -425,7 +442,7 @@ part_round_kernel(PartRound A) {
     uint32_t* hval = cnt + H;
     uint32_t* hlist = cnt + 2u * H;
     if (A.gen_slots) {
-      for (uint32_t i = tid; i < 2u * A.vbig + 1u; i += kPBlock) cnt[i] = 0u;
+      for (uint32_t i = tid; i < area; i += kPBlock) cnt[i] = 0u;
     } else {
       for (uint32_t i = tid; i < H; i += kPBlock) {
         hkey[i] = kNone;
@@ -471,7 +488,9 @@ part_round_kernel(PartRound A) {
             atomicAdd(&cnt[(second ? r0.y : 0u) + k], 1u);
           }
         }
-        step_and_scatter<false>(L, A, 1u, nullptr, 0u, cur, sec, wid, valid, pending);
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j) cur[j] = valid[j] ? cur[j] : kNone;
+        step_and_scatter<false>(L, A, 1u, nullptr, 0u, cur, sec, wid, pending);
         // the barrier at the end of step_and_scatter follows every count
         const uint32_t ns0 = r0.y, nsl = r0.y + (sl1 != sl0 ? r1.y : 0u);
         for (uint32_t i = tid; i < nsl; i += kPBlock) {
@@ -506,7 +525,9 @@ part_round_kernel(PartRound A) {
             }
           }
         }
-        step_and_scatter<false>(L, A, A.gen_hops - 1u, nullptr, 0u, cur, sec, wid, valid, pending);
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j) cur[j] = valid[j] ? cur[j] : kNone;
+        step_and_scatter<false>(L, A, A.gen_hops - 1u, nullptr, 0u, cur, sec, wid, pending);
         // the barrier at the end of step_and_scatter follows every insert
         const uint32_t nh = L.hn;
         for (uint32_t i = tid; i < nh; i += kPBlock) {
@@ -519,7 +540,9 @@ part_round_kernel(PartRound A) {
         if (tid == 0) L.hn = 0u;
         lds_barrier();
       } else {
-        step_and_scatter<false>(L, A, 0u, nullptr, 0u, cur, sec, wid, valid, pending);
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j) cur[j] = valid[j] ? cur[j] : kNone;
+        step_and_scatter<false>(L, A, 0u, nullptr, 0u, cur, sec, wid, pending);
       }
     }
   } else {
@@ -529,7 +552,24 @@ part_round_kernel(PartRound A) {
       if (l >= nbp) continue;  // uniform per block
       const uint32_t lo = L.plo[p];
       const uint32_t V = L.plo[p + 1] - lo;
-      for (uint32_t i = tid; i < V; i += kPBlock) cnt[i] = 0u;
+      for (uint32_t i = tid; i < (kCnt16 ? (V + 1u) / 2u : V); i += kPBlock) cnt[i] = 0u;
+      // adds the partition's counts into this workgroup's copy and clears them
+      auto flush = [&]() {
+        if constexpr (kCnt16) {
+          for (uint32_t i = tid; i < (V + 1u) / 2u; i += kPBlock) {
+            const uint32_t x = cnt[i];
+            if (x & 0xFFFFu) copy[lo + 2u * i] += x & 0xFFFFu;
+            if (x >> 16) copy[lo + 2u * i + 1u] += x >> 16;
+            cnt[i] = 0u;
+          }
+        } else {
+          for (uint32_t i = tid; i < V; i += kPBlock) {
+            const uint32_t c = cnt[i];
+            if (c) copy[lo + i] += c;
+          }
+        }
+      };
+      uint32_t since_flush = 0;
       if constexpr (kMode == kMid)
         for (uint32_t i = tid; i <= V; i += kPBlock) ridx[i] = A.idx[lo + i];
       const uint32_t base = A.pstart_in[p];
@@ -544,33 +584,38 @@ part_round_kernel(PartRound A) {
         }
         lds_barrier();
         uint32_t cur[kPer], sec[kPer], wid[kPer];
-        bool valid[kPer];
-        load_tile(A, L.lblk[0], L.lfill[0], cur, sec, wid, valid);
+        load_tile(A, L.lblk[0], L.lfill[0], cur, sec, wid);
         for (uint32_t t = 0; t < nl; ++t) {
           // the next tile's records are in flight while this one is stepped
           uint32_t ncur[kPer], nsec[kPer], nwid[kPer];
-          bool nvalid[kPer];
           load_tile(A, t + 1 < nl ? L.lblk[t + 1] : kNone, t + 1 < nl ? L.lfill[t + 1] : 0u,
-                    ncur, nsec, nwid, nvalid);
+                    ncur, nsec, nwid);
+          if (kCnt16 && ++since_flush > kFlushTiles) {  // uniform
+            lds_barrier();
+            flush();
+            lds_barrier();
+            since_flush = 1;
+          }
 #pragma unroll
-          for (uint32_t j = 0; j < kPer; ++j)
-            if (valid[j]) atomicAdd(&cnt[cur[j] - lo], 1u);
+          for (uint32_t j = 0; j < kPer; ++j) {
+            if (cur[j] != kNone) {
+              const uint32_t o = cur[j] - lo;
+              if constexpr (kCnt16) atomicAdd(&cnt[o >> 1], 1u << ((o & 1u) << 4));
+              else atomicAdd(&cnt[o], 1u);
+            }
+          }
           if constexpr (kMode == kMid)
-            step_and_scatter<true>(L, A, A.round, ridx, lo, cur, sec, wid, valid, pending);
+            step_and_scatter<true>(L, A, A.round, ridx, lo, cur, sec, wid, pending);
 #pragma unroll
           for (uint32_t j = 0; j < kPer; ++j) {
             cur[j] = ncur[j];
             sec[j] = nsec[j];
             wid[j] = nwid[j];
-            valid[j] = nvalid[j];
           }
         }
       }
       lds_barrier();
-      for (uint32_t i = tid; i < V; i += kPBlock) {
-        const uint32_t c = cnt[i];
-        if (c) copy[lo + i] += c;
-      }
+      flush();
     }
   }
   if (pending) copy_out(L, A);
@@ -828,7 +873,8 @@ bool part_shape(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint64_t n_src, uint32_t n
   for (uint32_t p = 0; p < S->P; ++p)
     S->vbig = std::max(S->vbig, dg->part_plo[p + 1] - dg->part_plo[p]);
   // every workgroup resident, a multiple of 8 (the XCDs)
-  S->lds = sizeof(PartLds) + (2ull * S->vbig + 1) * sizeof(uint32_t);
+  S->lds = sizeof(PartLds) +
+           ((kCnt16 ? (S->vbig + 1ull) / 2 : S->vbig) + S->vbig + 1ull) * sizeof(uint32_t);
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
           &per_cu, reinterpret_cast<const void*>(&part_round_kernel<kMid>), kPBlock, S->lds) !=
@@ -871,7 +917,7 @@ int saba_part_prepare(tgtp_ctx* ctx, tgtp_dgraph* dg, uint64_t n_src, uint32_t d
   *use = false;
   if (depth == 0 || !part_env_on(dg)) return TGTP_OK;
   const uint64_t n = dg->n;
-  const uint32_t vmax = static_cast<uint32_t>(std::max(64, env_int("TGTP_WALK_PART_VMAX", 7168)));
+  const uint32_t vmax = static_cast<uint32_t>(std::max(64, env_int("TGTP_WALK_PART_VMAX", kCnt16 ? 10240 : 8192)));
   const uint32_t want = static_cast<uint32_t>(std::max(1, env_int("TGTP_WALK_PARTS", 128)));
   const bool required = env_int("TGTP_WALK_PART", -1) > 0;
   auto refuse = [&](const char* why) {
@@ -992,7 +1038,8 @@ int launch_saba_walks_part(tgtp_ctx* ctx, tgtp_dgraph* dg, uint32_t src_begin, u
   A.counters = d_counters;
   A.gen_hops = std::min<uint32_t>(depth, static_cast<uint32_t>(
                                            std::max(1, env_int("TGTP_WALK_PART_GEN_HOPS", 2))));
-  A.gen_slots = A.gen_hops == 2u && n_walks >= kBR && 2ull * dg->max_deg <= 2ull * S.vbig + 1 &&
+  A.gen_slots = A.gen_hops == 2u && n_walks >= kBR &&
+                2ull * dg->max_deg <= (kCnt16 ? (S.vbig + 1ull) / 2 : S.vbig) + S.vbig + 1 &&
                 env_int("TGTP_WALK_PART_GEN_SLOTS", 1) != 0;
   A.cwd = cwd;
   A.reg_start = reg_start;
