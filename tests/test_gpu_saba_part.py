@@ -19,10 +19,14 @@ from tgtp_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture
-def part(monkeypatch):
+@pytest.fixture(params=["1", "0"], ids=["tagged", "lookup"])
+def part(monkeypatch, request):
+    """Engine required; the neighbours' partitions read from the tagged
+    adjacency (the default up to 2^24 vertices) or looked up in the partition
+    bounds (the path of larger graphs)."""
     monkeypatch.setenv("TGTP_WALK_PART", "1")
     monkeypatch.setenv("TGTP_WALK_PART_VMAX", "64")
+    monkeypatch.setenv("TGTP_WALK_PART_TAGS", request.param)
     return monkeypatch
 
 
@@ -160,6 +164,8 @@ def test_engines_agree_at_full_scale(gpu_ctx, monkeypatch):
     dg = gpu_ctx.upload(g)
     try:
         c = gpu_ctx.saba_walks(dg, 10, 16384, 200000, 200256)
+        monkeypatch.setenv("TGTP_WALK_PART_TAGS", "0")
+        d = gpu_ctx.saba_walks(dg, 10, 16384, 200000, 200256)
     finally:
         dg.free()
-    assert np.array_equal(c[0], ref)
+    assert np.array_equal(c[0], ref) and np.array_equal(d[0], ref)

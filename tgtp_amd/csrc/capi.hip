@@ -196,7 +196,7 @@ extern "C" void tgtp_graph_free(tgtp_ctx* ctx, tgtp_dgraph* dg) {
   (void)hipSetDevice(dg->device);
   if (dg->prep && dg->prep_free) dg->prep_free(dg->prep);
   void* ptrs[] = {dg->idx, dg->adj, dg->rowd, dg->rev, dg->degf, dg->lam, dg->X, dg->walk_adj,
-                   dg->part_dev};
+                   dg->part_dev, dg->part_adj};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete dg;

@@ -59,6 +59,10 @@ struct tgtp_dgraph {
   std::vector<uint32_t> part_plo;
   uint32_t part_vmax = 0, part_want = 0, part_tshift = 0;
   void* part_dev = nullptr;
+  // adjacency with each neighbour's partition in bits 24-31 (n <= 2^24),
+  // valid for the current partitions when part_tagged
+  uint32_t* part_adj = nullptr;
+  bool part_tagged = false;
   // tgtp host-side state (host CSR copy, symmetrization map), built lazily
   void* prep = nullptr;
   void (*prep_free)(void*) = nullptr;

@@ -94,6 +94,7 @@ struct PartRound {
   const uint32_t* adj;
   const uint32_t* plo;   // [P+1] partition vertex bounds
   const uint16_t* ptab;  // [kTab+2]
+  const uint32_t* tadj;  // adjacency tagged with partitions (vertex | p << 24), or null
   uint64_t n;
   uint32_t P;
   uint32_t vbig;         // vertices of the largest partition
@@ -132,9 +133,7 @@ struct PartRound {
 struct PartLds {
   uint32_t hist[kMaxParts];
   uint32_t off[kMaxParts];
-  uint32_t len0[kMaxParts];   // records of the tile's run that go to the current block
-  uint32_t seg0[kMaxParts];   // record index of the run's start in the current block
-  uint32_t seg1[kMaxParts];   // record index of the spill into a new block (kNone: none)
+  uint4 run[kMaxParts];
   uint32_t curblk[kMaxParts];
   uint32_t curfill[kMaxParts];
   uint32_t lcount[kMaxParts]; // blocks this workgroup wrote per partition
@@ -243,17 +242,8 @@ __device__ __forceinline__ void count_arrival(PartLds& L, uint32_t* hkey, uint32
 __device__ __forceinline__ void copy_out(PartLds& L, const PartRound& A) {
   const uint32_t total = L.total;
   for (uint32_t q = threadIdx.x; q < total; q += kPBlock) {
-    const uint32_t d = L.tdest[q];
-    const uint32_t jj = q - L.off[d];
-    const uint32_t l0 = L.len0[d];
-    uint32_t g;
-    if (jj < l0) {
-      g = L.seg0[d] + jj;
-    } else {
-      const uint32_t s1 = L.seg1[d];
-      if (s1 == kNone) continue;  // region overflow: reported, records dropped
-      g = s1 + (jj - l0);
-    }
+    const uint4 rn = L.run[L.tdest[q]];
+    const uint32_t g = q + (q < rn.x ? rn.y : rn.z);
     uint32_t* r = A.rec_out + static_cast<uint64_t>(g) * 3u;
     __builtin_nontemporal_store(L.tcur[q], r);
     __builtin_nontemporal_store(L.tsec[q], r + 1);
@@ -296,27 +286,30 @@ __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A,
       if (ns == 0u && sec[j] != 0u && hop + 1u < A.depth) mark_bouquet(A.taint, wid[j]);
       sec[j] = ns;
     }
-    nxt[j] = valid[j] ? A.adj[rd[j].x + k] : 0u;
+    nxt[j] = valid[j] ? (A.tadj ? A.tadj : A.adj)[rd[j].x + k] : 0u;
   }
   if (pending) copy_out(L, A);
+  if (A.tadj) {  // uniform
 #pragma unroll
-  for (uint32_t j = 0; j < kPer; ++j) {
-    dest[j] = 0u;
-    rank[j] = 0u;
-    if (valid[j]) {
-      dest[j] = part_of(L, nxt[j], A.tshift);
-      rank[j] = atomicAdd(&L.hist[dest[j]], 1u);
+    for (uint32_t j = 0; j < kPer; ++j) {
+      dest[j] = nxt[j] >> 24;
+      nxt[j] &= 0xFFFFFFu;
     }
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) dest[j] = valid[j] ? part_of(L, nxt[j], A.tshift) : 0u;
   }
+#pragma unroll
+  for (uint32_t j = 0; j < kPer; ++j) rank[j] = valid[j] ? atomicAdd(&L.hist[dest[j]], 1u) : 0u;
   lds_barrier();
   scan_bins(L);
   if (tid < A.P) {
     const uint32_t c = L.hist[tid];
     uint32_t cb = L.curblk[tid], cf = L.curfill[tid];
     const uint32_t l0 = min(kBR - cf, c);
-    L.len0[tid] = l0;
-    L.seg0[tid] = cb == kNone ? 0u : cb * kBR + cf;
-    L.seg1[tid] = kNone;
+    const uint32_t o = L.off[tid];
+    const uint32_t seg0 = cb == kNone ? 0u : cb * kBR + cf;
+    uint32_t seg1 = A.pool_cap * kBR;
     if (c > l0) {
       if (cb != kNone) A.bfill_out[cb] = kBR;  // full
       const uint32_t i = atomicAdd(&L.used, 1u);
@@ -324,7 +317,7 @@ __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A,
       if (i < L.rsize && nb < A.pool_cap) {
         A.bpart_out[nb] = tid;
         A.bslot_out[nb] = L.lcount[tid]++;
-        L.seg1[tid] = nb * kBR;
+        seg1 = nb * kBR;
         cb = nb;
         cf = c - l0;
       } else {
@@ -335,6 +328,7 @@ __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A,
     } else {
       cf += l0;
     }
+    L.run[tid] = make_uint4(o + l0, seg0 - o, seg1 - o - l0, 0u);
     L.curblk[tid] = cb;
     L.curfill[tid] = cf;
   }
@@ -701,6 +695,25 @@ part_order_kernel(const uint32_t* __restrict__ bcount, const uint32_t* __restric
   }
 }
 
+// out[e] = adj[e] | (partition of adj[e]) << 24; vertices below 2^24.
+__global__ void __launch_bounds__(256)
+part_tag_kernel(const uint32_t* __restrict__ adj, uint64_t m, const uint32_t* __restrict__ plo,
+                uint32_t P, uint32_t* __restrict__ out) {
+  __shared__ uint32_t s_plo[kMaxParts + 1];
+  for (uint32_t i = threadIdx.x; i <= P; i += 256u) s_plo[i] = plo[i];
+  __syncthreads();
+  for (uint64_t e = blockIdx.x * 256ull + threadIdx.x; e < m; e += gridDim.x * 256ull) {
+    const uint32_t v = adj[e];
+    uint32_t a = 0, b = P - 1u;  // largest p with plo[p] <= v
+    while (a < b) {
+      const uint32_t mid = (a + b + 1u) >> 1;
+      if (s_plo[mid] <= v) a = mid;
+      else b = mid - 1u;
+    }
+    out[e] = v | (a << 24);
+  }
+}
+
 __global__ void part_fill_kernel(uint32_t* __restrict__ p, uint64_t n) {
   for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
@@ -905,7 +918,7 @@ bool part_shape(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint64_t n_src, uint32_t n
   S->chunk_src = (n_src + S->n_chunks - 1) / S->n_chunks;
   const uint64_t walkers = S->chunk_src * n_walks;
   const uint64_t cap = (walkers + kBR - 1) / kBR + slack_blocks;
-  if (cap > kMaxPoolBlocks) return false;
+  if (cap + 1 > kMaxPoolBlocks) return false;
   S->pool_cap = static_cast<uint32_t>(cap);
   return true;
 }
@@ -927,6 +940,7 @@ int saba_part_prepare(tgtp_ctx* ctx, tgtp_dgraph* dg, uint64_t n_src, uint32_t d
   };
   if (dg->part_plo.empty() || dg->part_vmax != vmax || dg->part_want != want) {
     dg->part_plo.clear();
+    dg->part_tagged = false;
     std::vector<uint2> rd(n);
     TGTP_HIP_CHECK(ctx, hipMemcpy(rd.data(), dg->rowd, n * sizeof(uint2), hipMemcpyDeviceToHost));
     std::vector<uint32_t> deg(n);
@@ -956,6 +970,19 @@ int saba_part_prepare(tgtp_ctx* ctx, tgtp_dgraph* dg, uint64_t n_src, uint32_t d
     dg->part_vmax = vmax;
     dg->part_want = want;
     dg->part_tshift = tshift;
+    dg->part_tagged = false;
+  }
+  // the neighbour gathers of the rounds read the partition with the vertex
+  // (TGTP_WALK_PART_TAGS=0: looked up in the boundaries instead)
+  if (n <= (1ull << 24) && env_int("TGTP_WALK_PART_TAGS", 1) != 0 && !dg->part_tagged) {
+    if (!dg->part_adj)
+      TGTP_HIP_CHECK(ctx, hipMalloc(&dg->part_adj, std::max<uint64_t>(dg->m, 1) * sizeof(uint32_t)));
+    part_tag_kernel<<<static_cast<uint32_t>(std::min<uint64_t>(8192, (dg->m + 255) / 256 + 1)), 256,
+                      0, ctx->stream>>>(dg->adj, dg->m, static_cast<const uint32_t*>(dg->part_dev),
+                                        static_cast<uint32_t>(dg->part_plo.size() - 1),
+                                        dg->part_adj);
+    TGTP_HIP_CHECK(ctx, hipGetLastError());
+    dg->part_tagged = true;
   }
   PartShape S;
   *use = part_shape(ctx, dg, n_src, n_walks, &S);
@@ -978,7 +1005,7 @@ int launch_saba_walks_part(tgtp_ctx* ctx, tgtp_dgraph* dg, uint32_t src_begin, u
 
   // buffers: two record pools with their block tables, the per-round
   // region tables, counter copies, bouquet bitmap
-  const uint64_t rec_bytes = static_cast<uint64_t>(pool_cap) * kBR * 12u;
+  const uint64_t rec_bytes = (static_cast<uint64_t>(pool_cap) + 1u) * kBR * 12u;
   // per pool: bfill, bpart, bslot, order [pool_cap] + bcount, pstart [kMaxParts]
   const uint64_t meta_words = 4ull * pool_cap + 2ull * kMaxParts;
   // shared: cwd, off [kMaxParts * grid] + reg_start, reg_used [grid] + overflow
@@ -1027,6 +1054,7 @@ int launch_saba_walks_part(tgtp_ctx* ctx, tgtp_dgraph* dg, uint32_t src_begin, u
   A.plo = static_cast<const uint32_t*>(dg->part_dev);
   A.ptab = reinterpret_cast<const uint16_t*>(static_cast<const uint8_t*>(dg->part_dev) +
                                              (kMaxParts + 1) * sizeof(uint32_t));
+  A.tadj = dg->part_tagged && env_int("TGTP_WALK_PART_TAGS", 1) != 0 ? dg->part_adj : nullptr;
   A.n = n;
   A.P = P;
   A.tshift = dg->part_tshift;
