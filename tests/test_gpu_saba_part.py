@@ -87,6 +87,7 @@ def test_walker_stops_at_vertex_without_neighbours(gpu_ctx, part):
     {"TGTP_WALK_PART_GEN_HOPS": "1"},                   # one hop in the generation round
     {"TGTP_WALK_PART_GEN_HOPS": "4"},                   # four
     {"TGTP_WALK_PART_VMAX": "512"},                     # arrival table of 256 slots
+    {"TGTP_WALK_PART_BALANCE": "0"},                    # partition p to workgroups b = p mod 8
 ])
 def test_engine_shapes(gpu_ctx, part, knobs):
     for k, v in knobs.items():

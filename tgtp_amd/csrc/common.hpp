@@ -57,7 +57,7 @@ struct tgtp_dgraph {
   // vertex partitions of the partitioned walk engine (saba_part.hip): bounds
   // on the host, bounds and the coarse lookup table on the device
   std::vector<uint32_t> part_plo;
-  uint32_t part_vmax = 0, part_want = 0, part_tshift = 0;
+  uint32_t part_vmax = 0, part_want = 0, part_tshift = 0, part_balance = 0;
   void* part_dev = nullptr;
   // adjacency with each neighbour's partition in bits 24-31 (n <= 2^24),
   // valid for the current partitions when part_tagged

@@ -95,6 +95,7 @@ struct PartRound {
   const uint32_t* plo;   // [P+1] partition vertex bounds
   const uint16_t* ptab;  // [kTab+2]
   const uint32_t* tadj;  // adjacency tagged with partitions (vertex | p << 24), or null
+  const uint8_t* pxcd;   // [kMaxParts] workgroups b with b % 8 == pxcd[p] serve partition p
   uint64_t n;
   uint32_t P;
   uint32_t vbig;         // vertices of the largest partition
@@ -145,6 +146,7 @@ struct PartLds {
   uint32_t hn;                // generation round: entries in the arrival table
   uint32_t plo[kMaxParts + 1];
   uint16_t ptab[kTab + 2];
+  uint8_t pxcd[kMaxParts];
   uint32_t lblk[kList];       // this workgroup's input blocks of the current partition
   uint32_t lfill[kList];
   uint32_t tcur[kBR];
@@ -383,6 +385,7 @@ part_round_kernel(PartRound A) {
   const uint32_t NL = gridDim.x >> 3;
   for (uint32_t i = tid; i <= A.P; i += kPBlock) L.plo[i] = A.plo[i];
   for (uint32_t i = tid; i < kTab + 2; i += kPBlock) L.ptab[i] = A.ptab[i];
+  if (tid < kMaxParts) L.pxcd[tid] = A.pxcd[tid];
   if (tid < kMaxParts) {
     L.hist[tid] = 0u;
     L.curblk[tid] = kNone;
@@ -404,7 +407,7 @@ part_round_kernel(PartRound A) {
     uint32_t below = 0, mine = 0;
     if (tid < A.P) {
       const uint32_t nbp = A.bcount_in[tid];
-      const uint32_t xp = tid & 7u;
+      const uint32_t xp = L.pxcd[tid];
       const uint32_t Lp = w > xp ? (w - xp + 7u) >> 3 : 0u;  // l' with 8 l' + xp < w
       below = (nbp / NL) * Lp + min(nbp % NL, Lp);
       if (xp == xc) mine = nbp / NL + (l < nbp % NL ? 1u : 0u);
@@ -541,7 +544,8 @@ part_round_kernel(PartRound A) {
     }
   } else {
     uint32_t* copy = A.copies + static_cast<uint64_t>(l) * A.n;
-    for (uint32_t p = xc; p < A.P; p += 8u) {
+    for (uint32_t p = 0; p < A.P; ++p) {
+      if (L.pxcd[p] != xc) continue;  // uniform
       const uint32_t nbp = A.bcount_in[p];
       if (l >= nbp) continue;  // uniform per block
       const uint32_t lo = L.plo[p];
@@ -861,6 +865,35 @@ bool plan_parts(const std::vector<uint32_t>& deg, uint32_t vmax, uint32_t want,
   return false;
 }
 
+// XCD of each partition (the residue mod 8 of the workgroups serving it).
+// Balanced: heaviest first to the XCD with the least expected records so
+// far, a partition weighing its degree sum (a walk visits a vertex at a rate
+// proportional to its degree once it has mixed); each XCD takes its
+// partitions in increasing order.  Otherwise p % 8.
+std::vector<uint8_t> assign_xcds(const std::vector<uint32_t>& deg,
+                                 const std::vector<uint32_t>& plo, bool balance) {
+  const uint32_t P = static_cast<uint32_t>(plo.size() - 1);
+  std::vector<uint8_t> x(kMaxParts, 0);
+  for (uint32_t p = 0; p < P; ++p) x[p] = static_cast<uint8_t>(p & 7u);
+  if (!balance) return x;
+  std::vector<double> w(P, 0.0);
+  for (uint32_t p = 0; p < P; ++p)
+    for (uint32_t v = plo[p]; v < plo[p + 1]; ++v) w[p] += deg[v];
+  std::vector<uint32_t> order(P);
+  for (uint32_t p = 0; p < P; ++p) order[p] = p;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t a, uint32_t b) { return w[a] > w[b]; });
+  double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t p : order) {
+    uint32_t best = 0;
+    for (uint32_t k = 1; k < 8; ++k)
+      if (load[k] < load[best]) best = k;
+    x[p] = static_cast<uint8_t>(best);
+    load[best] += w[p];
+  }
+  return x;
+}
+
 struct PartShape {
   uint32_t P = 0, vbig = 0, grid = 0, NL = 0, pool_cap = 0;
   size_t lds = 0;
@@ -938,7 +971,10 @@ int saba_part_prepare(tgtp_ctx* ctx, tgtp_dgraph* dg, uint64_t n_src, uint32_t d
                                 std::string("walk: partitioned engine required but ") + why)
                     : TGTP_OK;
   };
-  if (dg->part_plo.empty() || dg->part_vmax != vmax || dg->part_want != want) {
+  // TGTP_WALK_PART_BALANCE=0: partition p to the workgroups b with b % 8 == p % 8
+  const uint32_t balance = env_int("TGTP_WALK_PART_BALANCE", 1) != 0 ? 1u : 0u;
+  if (dg->part_plo.empty() || dg->part_vmax != vmax || dg->part_want != want ||
+      dg->part_balance != balance) {
     dg->part_plo.clear();
     dg->part_tagged = false;
     std::vector<uint2> rd(n);
@@ -958,8 +994,13 @@ int saba_part_prepare(tgtp_ctx* ctx, tgtp_dgraph* dg, uint64_t n_src, uint32_t d
     }
     if (!dg->part_dev) {
       TGTP_HIP_CHECK(ctx, hipMalloc(&dg->part_dev, (kMaxParts + 1) * sizeof(uint32_t) +
-                                                       (kTab + 2) * sizeof(uint16_t)));
+                                                       (kTab + 2) * sizeof(uint16_t) + kMaxParts));
     }
+    const std::vector<uint8_t> pxcd = assign_xcds(deg, plo, balance != 0);
+    TGTP_HIP_CHECK(ctx, hipMemcpy(static_cast<uint8_t*>(dg->part_dev) +
+                                      (kMaxParts + 1) * sizeof(uint32_t) +
+                                      (kTab + 2) * sizeof(uint16_t),
+                                  pxcd.data(), kMaxParts, hipMemcpyHostToDevice));
     TGTP_HIP_CHECK(ctx, hipMemcpy(dg->part_dev, plo.data(), plo.size() * sizeof(uint32_t),
                                   hipMemcpyHostToDevice));
     TGTP_HIP_CHECK(ctx, hipMemcpy(static_cast<uint8_t*>(dg->part_dev) +
@@ -970,6 +1011,7 @@ int saba_part_prepare(tgtp_ctx* ctx, tgtp_dgraph* dg, uint64_t n_src, uint32_t d
     dg->part_vmax = vmax;
     dg->part_want = want;
     dg->part_tshift = tshift;
+    dg->part_balance = balance;
     dg->part_tagged = false;
   }
   // the neighbour gathers of the rounds read the partition with the vertex
@@ -1054,6 +1096,8 @@ int launch_saba_walks_part(tgtp_ctx* ctx, tgtp_dgraph* dg, uint32_t src_begin, u
   A.plo = static_cast<const uint32_t*>(dg->part_dev);
   A.ptab = reinterpret_cast<const uint16_t*>(static_cast<const uint8_t*>(dg->part_dev) +
                                              (kMaxParts + 1) * sizeof(uint32_t));
+  A.pxcd = static_cast<const uint8_t*>(dg->part_dev) + (kMaxParts + 1) * sizeof(uint32_t) +
+           (kTab + 2) * sizeof(uint16_t);
   A.tadj = dg->part_tagged && env_int("TGTP_WALK_PART_TAGS", 1) != 0 ? dg->part_adj : nullptr;
   A.n = n;
   A.P = P;
