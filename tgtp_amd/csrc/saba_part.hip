@@ -931,8 +931,20 @@ bool part_shape(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint64_t n_src, uint32_t n
   S->grid = std::max<uint32_t>(8u, static_cast<uint32_t>(ctx->num_cus * per_cu) / 8u * 8u);
   S->NL = S->grid / 8u;
   // chunk: sources whose walkers fit the record pools
-  const uint64_t budget = static_cast<uint64_t>(std::max(64, env_int("TGTP_WALK_PART_MIB", 49152)))
-                          << 20;
+  // Fewer, larger chunks save rounds, and each round costs a fixed share
+  // (a partial block per partition and workgroup, the tail of the launch):
+  // the pools may take up to 96 GiB, where the 32-bit record index ends, but
+  // at most 3/4 of the memory free beside what they hold already (the free
+  // memory is not queried once they hold the cap).
+  const uint64_t pool_cap_bytes =
+      static_cast<uint64_t>(std::max(64, env_int("TGTP_WALK_PART_MIB", 98304))) << 20;
+  const uint64_t have = ctx->bufs[kBufPartRecA].bytes + ctx->bufs[kBufPartRecB].bytes;
+  uint64_t budget = pool_cap_bytes;
+  if (have < pool_cap_bytes) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return false;
+    budget = std::min<uint64_t>(pool_cap_bytes, std::max<uint64_t>(have, (free_b + have) / 4 * 3));
+  }
   // a round's regions hold its input blocks plus P per workgroup, and its
   // input holds at most the walkers' full blocks plus P partial blocks per
   // workgroup of the round before
