@@ -114,7 +114,7 @@ def bench_saba(args, g=None, depth=None, steps=None, warmup=None):
             and 4 * g.m + 8 * g.n >= 64 * 2**20)
     if part:
         kernel = ("part_round_kernel<0,1,2> + part_fixup_kernel + block listing (one pass; "
-                  "the middle rounds part_round_kernel<1> are 90% of it)")
+                  "the middle rounds part_round_kernel<1> are 84% of it)")
         traffic = traffic_for("saba_part_pass", f"cl1m_d{depth}_w{n_walks}")
     else:
         kernel = "saba_walk_kernel + bucket_count_kernel (one pass)"
