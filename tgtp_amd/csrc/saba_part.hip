@@ -241,12 +241,22 @@ __device__ __forceinline__ void count_arrival(PartLds& L, uint32_t* hkey, uint32
 // row offsets ridx[0, V] (vertices lo + i) are staged in LDS, so the hop's only
 // gather is the neighbour itself; otherwise the row descriptors are gathered.
 // Writes the sorted tile held in LDS to its blocks.
+//
+// Every thread issues the same kPer stores whatever the tile's size: slots past
+// it write to the pool's spare block (block pool_cap, never read as records),
+// so no branch or loop of data-dependent length stands between the
+// neighbour gathers and their use, and the compiler's wait before that use
+// counts exactly the loads and stores issued after the gathers instead of
+// waiting for every outstanding access.
 __device__ __forceinline__ void copy_out(PartLds& L, const PartRound& A) {
   const uint32_t total = L.total;
-  for (uint32_t q = threadIdx.x; q < total; q += kPBlock) {
+  const uint64_t spare = static_cast<uint64_t>(A.pool_cap) * kBR + threadIdx.x;
+#pragma unroll
+  for (uint32_t j = 0; j < kPer; ++j) {
+    const uint32_t q = j * kPBlock + threadIdx.x;
     const uint4 rn = L.run[L.tdest[q]];
     const uint32_t g = q + (q < rn.x ? rn.y : rn.z);
-    uint32_t* r = A.rec_out + static_cast<uint64_t>(g) * 3u;
+    uint32_t* r = A.rec_out + (q < total ? static_cast<uint64_t>(g) : spare) * 3u;
     __builtin_nontemporal_store(L.tcur[q], r);
     __builtin_nontemporal_store(L.tsec[q], r + 1);
     __builtin_nontemporal_store(L.twid[q], r + 2);
@@ -256,7 +266,10 @@ __device__ __forceinline__ void copy_out(PartLds& L, const PartRound& A) {
 // `pending`: LDS holds the previous tile, sorted; its records are written
 // after this tile's neighbour gathers are issued.  Vector-memory completions
 // are counted in issue order, so stores issued before the gathers would hold
-// up the wait for them.  The caller writes the last tile (copy_out).
+// up the wait for them.  The caller writes the last tile (copy_out).  (The
+// next tile's record loads are issued at the top of the tile, before the
+// gathers; issued after them -- the next tile's or the one after's -- the
+// pass was 8-10% slower.)
 template <bool kLdsRows>
 __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A, uint32_t hop,
                                                  const uint32_t* ridx, uint32_t lo,
@@ -266,31 +279,34 @@ __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A,
   bool valid[kPer];  // kNone marks an empty slot
 #pragma unroll
   for (uint32_t j = 0; j < kPer; ++j) valid[j] = cur[j] != kNone;
+  // (branch-free: empty slots read row 0 / entry 0, and their results are
+  // dropped; a bouquet to mark is marked once the gathers have been used)
   uint2 rd[kPer];
 #pragma unroll
   for (uint32_t j = 0; j < kPer; ++j) {
     if constexpr (kLdsRows) {
-      const uint32_t a = valid[j] ? ridx[cur[j] - lo] : 0u;
-      const uint32_t b = valid[j] ? ridx[cur[j] - lo + 1u] : 0u;
-      rd[j] = make_uint2(a, b - a);
+      const uint32_t o = valid[j] ? cur[j] - lo : 0u;
+      const uint32_t a = ridx[o];
+      rd[j] = make_uint2(a, ridx[o + 1u] - a);
     } else {
-      rd[j] = valid[j] ? A.rowd[cur[j]] : make_uint2(0u, 0u);
+      rd[j] = A.rowd[valid[j] ? cur[j] : 0u];
     }
   }
   uint32_t nxt[kPer], dest[kPer], rank[kPer];
+  bool mark[kPer];
+  const uint32_t* gadj = A.tadj ? A.tadj : A.adj;
 #pragma unroll
   for (uint32_t j = 0; j < kPer; ++j) {
     valid[j] = valid[j] && rd[j].y != 0u;
-    uint32_t k = 0u;
-    if (valid[j]) {
-      k = __umulhi(sec[j] ^ fmix32(cur[j]), rd[j].y);
-      const uint32_t ns = sec[j] * rd[j].y;
-      if (ns == 0u && sec[j] != 0u && hop + 1u < A.depth) mark_bouquet(A.taint, wid[j]);
-      sec[j] = ns;
-    }
-    nxt[j] = valid[j] ? (A.tadj ? A.tadj : A.adj)[rd[j].x + k] : 0u;
+    const uint32_t k = __umulhi(sec[j] ^ fmix32(cur[j]), rd[j].y);
+    const uint32_t ns = sec[j] * rd[j].y;
+    mark[j] = valid[j] && ns == 0u && sec[j] != 0u && hop + 1u < A.depth;
+    sec[j] = valid[j] ? ns : sec[j];
+    nxt[j] = gadj[valid[j] ? rd[j].x + k : 0u];
   }
-  if (pending) copy_out(L, A);
+  // keeps the gathers ahead of the stores in issue order
+  __asm__ volatile("" ::: "memory");
+  copy_out(L, A);  // the previous tile (none yet: L.total is 0)
   if (A.tadj) {  // uniform
 #pragma unroll
     for (uint32_t j = 0; j < kPer; ++j) {
@@ -301,6 +317,9 @@ __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A,
 #pragma unroll
     for (uint32_t j = 0; j < kPer; ++j) dest[j] = valid[j] ? part_of(L, nxt[j], A.tshift) : 0u;
   }
+#pragma unroll
+  for (uint32_t j = 0; j < kPer; ++j)
+    if (mark[j]) mark_bouquet(A.taint, wid[j]);  // rare
 #pragma unroll
   for (uint32_t j = 0; j < kPer; ++j) rank[j] = valid[j] ? atomicAdd(&L.hist[dest[j]], 1u) : 0u;
   lds_barrier();
@@ -350,21 +369,24 @@ __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A,
 }
 
 // Loads input record i of block blk (kNone: none) as vertex / secret / id;
-// slots past the block's fill get vertex kNone.
+// slots past the block's fill get vertex kNone (branch-free: they read the
+// pool's spare block, and the values are dropped).
 __device__ __forceinline__ void load_tile(const PartRound& A, uint32_t blk, uint32_t fill,
                                           uint32_t (&cur)[kPer], uint32_t (&sec)[kPer],
                                           uint32_t (&wid)[kPer]) {
 #pragma unroll
   for (uint32_t j = 0; j < kPer; ++j) {
     const uint32_t i = j * kPBlock + threadIdx.x;
-    cur[j] = kNone;
-    sec[j] = wid[j] = 0u;
-    if (blk != kNone && i < fill) {
-      const uint32_t* r = A.rec_in + (static_cast<uint64_t>(blk) * kBR + i) * 3u;
-      cur[j] = __builtin_nontemporal_load(r);
-      sec[j] = __builtin_nontemporal_load(r + 1);
-      wid[j] = __builtin_nontemporal_load(r + 2);
-    }
+    const bool ok = blk != kNone && i < fill;
+    const uint64_t g = ok ? static_cast<uint64_t>(blk) * kBR + i
+                          : static_cast<uint64_t>(A.pool_cap) * kBR + threadIdx.x;
+    const uint32_t* r = A.rec_in + g * 3u;
+    const uint32_t c = __builtin_nontemporal_load(r);
+    const uint32_t x = __builtin_nontemporal_load(r + 1);
+    const uint32_t y = __builtin_nontemporal_load(r + 2);
+    cur[j] = ok ? c : kNone;
+    sec[j] = ok ? x : 0u;
+    wid[j] = ok ? y : 0u;
   }
 }
 
@@ -392,7 +414,10 @@ part_round_kernel(PartRound A) {
     L.curfill[tid] = kBR;
     L.lcount[tid] = 0u;
   }
-  if (tid == 0) L.used = 0u;
+  if (tid == 0) {
+    L.used = 0u;
+    L.total = 0u;  // no tile to write out yet
+  }
   // this workgroup's output region: the input blocks of workgroups before it
   // plus P each, then its own input blocks plus P
   if constexpr (kMode == kGen) {
