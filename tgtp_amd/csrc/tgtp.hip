@@ -234,6 +234,14 @@ struct SrcArgs {
   float* tr_x;
 };
 
+// Entries between a pipeline stage of the push's frontier loop and the stage
+// it depends on (a power of two).
+#ifndef TGTP_PIPE_GAP
+#define TGTP_PIPE_GAP 2
+#endif
+constexpr uint32_t kPipeGap = TGTP_PIPE_GAP;
+static_assert((kPipeGap & (kPipeGap - 1u)) == 0u, "a power of two");
+
 template <bool kLds>
 __device__ void faithful_source(const SrcArgs& A, uint32_t src, float* pvec, uint8_t* seen,
                                 uint32_t* qa, uint32_t* qb) {
@@ -277,74 +285,105 @@ __device__ void faithful_source(const SrcArgs& A, uint32_t src, float* pvec, uin
     // ---- diffuse: aesc.h:222-234, frontier in order
     uint32_t qn_size = 0;
     // The queue, row descriptors, adjacency and receiver degrees do not
-    // depend on pvec: a four-stage pipeline loads, for frontier entries
-    // i+4 .. i+1, the vertex id, its row descriptor, this lane's neighbour in
-    // the row's first 64 entries, and that neighbour's degree, one stage per
-    // entry and iteration, so no entry waits on its own chain of loads.
-    // Entries past the frontier repeat the source (a valid vertex).
-    auto qv = [&](uint32_t k) { return k < qsize ? q[k] : src; };
-    auto first_nb = [&](uint2 r) { return ln < r.y ? A.adj[r.x + ln] : 0xFFFFFFFFu; };
-    auto nb_deg = [&](uint32_t u) { return u != 0xFFFFFFFFu ? A.degf[u] : 1.0f; };
-    uint32_t V1 = qv(1), V2 = qv(2), V3 = qv(3);
-    uint2 R0 = A.rowd[qv(0)], R1 = A.rowd[V1], R2 = A.rowd[V2];
-    uint32_t V0 = qv(0);
-    uint32_t U0 = first_nb(R0), U1 = first_nb(R1);
-    float G0 = nb_deg(U0);
-    for (uint32_t i = 0; i < qsize; ++i) {
-      const uint32_t v = V0;
-      const uint2 rv = R0;
-      const uint32_t u_first = U0;
-      const float g_first = G0;
-      {  // next entries' stages
-        const float G1 = nb_deg(U1);
-        const uint32_t U2 = first_nb(R2);
-        const uint2 R3 = A.rowd[V3];
-        const uint32_t V4 = qv(i + 4);
-        V0 = V1;
-        V1 = V2;
-        V2 = V3;
-        V3 = V4;
-        R0 = R1;
-        R1 = R2;
-        R2 = R3;
-        U0 = U1;
-        U1 = U2;
-        G0 = G1;
+    // depend on pvec: a four-stage pipeline loads, for frontier entries ahead
+    // of the current one, the vertex id, its row descriptor, this lane's
+    // neighbour in the row's first 64 entries and that neighbour's degree,
+    // each stage kPipeGap entries after the one it depends on, so a step waits
+    // only for loads issued kPipeGap steps earlier.  Entries past the frontier
+    // repeat the source (a valid vertex).  Entry e's stages live in slot
+    // e % kSlots and the loop is unrolled kSlots times, so that no register
+    // holding a load in flight is copied (a copy waits for the load), and the
+    // loads are branch-free (an out-of-range slot reads a valid word and
+    // drops it): the compiler's waits then count only the loads a step uses.
+    // The loaded words are used as they come (no select or copy touches a
+    // load in flight): an entry past the frontier reads q[0]; a lane past
+    // its row reads adj[0] and its degree degf[0], values the push ignores.
+    // (the queue index carries a zero the compiler cannot see as uniform, so
+    // the vertex ids and row descriptors in flight stay in vector registers
+    // until their step instead of being moved to scalar ones at once, which
+    // waits for the load)
+    uint32_t zv;
+    __asm__ volatile("v_mov_b32 %0, 0" : "=v"(zv));
+    auto qv = [&](uint32_t k) { return q[(k < qsize ? k : 0u) + zv]; };
+    auto first_nb = [&](uint2 r) { return A.adj[ln < r.y ? r.x + ln : 0u]; };
+    auto nb_deg = [&](uint2 r, uint32_t u) { return A.degf[ln < r.y ? u : 0u]; };
+    // one chunk of 64 row entries of frontier vertex v: u is this lane's
+    // entry (0xFFFFFFFF past the row), g its degree as a float
+    auto push_chunk = [&](const uint2 rv, uint32_t base, uint32_t u, float residue, float g) {
+      const uint32_t j = base + ln;
+      const bool valid = j < rv.y;
+      const uint32_t up = __shfl_up(u, 1);
+      const bool lead = valid && (ln == 0 || u != up);
+      bool newrun = lead;
+      if (ln == 0 && base > 0 && valid) newrun = (u != A.adj[rv.x + base - 1]);
+      const uint64_t leads = __ballot(lead);
+      const uint32_t nvalid = min(64u, rv.y - base);
+      uint32_t run = 0;
+      if (lead) {
+        const uint64_t higher = (ln == 63) ? 0ull : (leads >> (ln + 1));
+        run = higher ? static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(higher)))
+                     : (nvalid - ln);
       }
-      const float residue = pvec[v];
+      const bool fresh = newrun && !seen[u];
+      const uint64_t fm = __ballot(fresh);
+      if (fresh) {
+        qn[qn_size + __popcll(fm & lanes_below())] = u;
+        seen[u] = 1;
+      }
+      qn_size += __popcll(fm);
+      if (lead) {
+        const float c = residue / g;
+        float t = pvec[u];
+        for (uint32_t k = 0; k < run; ++k) t = t + c;
+        pvec[u] = t;
+      }
       vsync();
-      if (ln == 0) pvec[v] = 0.0f;
-      vsync();
-      for (uint32_t base = 0; base < rv.y; base += 64) {
-        const uint32_t j = base + ln;
-        const bool valid = j < rv.y;
-        const uint32_t u = base == 0 ? u_first : (valid ? A.adj[rv.x + j] : 0xFFFFFFFFu);
-        const uint32_t up = __shfl_up(u, 1);
-        const bool lead = valid && (ln == 0 || u != up);
-        bool newrun = lead;
-        if (ln == 0 && base > 0 && valid) newrun = (u != A.adj[rv.x + base - 1]);
-        const uint64_t leads = __ballot(lead);
-        const uint32_t nvalid = min(64u, rv.y - base);
-        uint32_t run = 0;
-        if (lead) {
-          const uint64_t higher = (ln == 63) ? 0ull : (leads >> (ln + 1));
-          run = higher ? static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(higher)))
-                       : (nvalid - ln);
+    };
+    constexpr uint32_t kSlots = 4u * kPipeGap;
+    uint32_t Vs[kSlots];
+    uint2 Rs[kSlots];
+    uint32_t Us[kSlots];
+    float Gs[kSlots];
+#pragma unroll
+    for (uint32_t e = 0; e < kSlots; ++e) {
+      Vs[e] = qv(e);
+      Rs[e] = make_uint2(0u, 0u);
+      Us[e] = 0u;
+      Gs[e] = 1.0f;
+    }
+#pragma unroll
+    for (uint32_t e = 0; e < 3u * kPipeGap; ++e) Rs[e] = A.rowd[Vs[e]];
+#pragma unroll
+    for (uint32_t e = 0; e < 2u * kPipeGap; ++e) Us[e] = first_nb(Rs[e]);
+#pragma unroll
+    for (uint32_t e = 0; e < kPipeGap; ++e) Gs[e] = nb_deg(Rs[e], Us[e]);
+    for (uint32_t i0 = 0; i0 < qsize; i0 += kSlots) {
+#pragma unroll
+      for (uint32_t e = 0; e < kSlots; ++e) {
+        const uint32_t i = i0 + e;
+        if (i < qsize) {  // uniform
+          const uint32_t v = __builtin_amdgcn_readfirstlane(Vs[e]);
+          const uint2 rv = make_uint2(__builtin_amdgcn_readfirstlane(Rs[e].x),
+                                      __builtin_amdgcn_readfirstlane(Rs[e].y));
+          const uint32_t u_first = ln < rv.y ? Us[e] : 0xFFFFFFFFu;
+          const float g_first = Gs[e];
+          // stages of entries i + kPipeGap .. i + 4 kPipeGap
+          constexpr uint32_t kM = kSlots - 1u;
+          Gs[(e + kPipeGap) & kM] = nb_deg(Rs[(e + kPipeGap) & kM], Us[(e + kPipeGap) & kM]);
+          Us[(e + 2u * kPipeGap) & kM] = first_nb(Rs[(e + 2u * kPipeGap) & kM]);
+          Rs[(e + 3u * kPipeGap) & kM] = A.rowd[Vs[(e + 3u * kPipeGap) & kM]];
+          Vs[e] = qv(i + kSlots);
+          const float residue = pvec[v];
+          vsync();
+          if (ln == 0) pvec[v] = 0.0f;
+          vsync();
+          if (rv.y != 0) push_chunk(rv, 0u, u_first, residue, g_first);
+          for (uint32_t base = 64; base < rv.y; base += 64) {
+            const uint32_t j = base + ln;
+            const uint32_t u = j < rv.y ? A.adj[rv.x + j] : 0xFFFFFFFFu;
+            push_chunk(rv, base, u, residue, j < rv.y ? A.degf[u] : 1.0f);
+          }
         }
-        const bool fresh = newrun && !seen[u];
-        const uint64_t fm = __ballot(fresh);
-        if (fresh) {
-          qn[qn_size + __popcll(fm & lanes_below())] = u;
-          seen[u] = 1;
-        }
-        qn_size += __popcll(fm);
-        if (lead) {
-          const float c = residue / (base == 0 ? g_first : A.degf[u]);
-          float t = pvec[u];
-          for (uint32_t k = 0; k < run; ++k) t = t + c;
-          pvec[u] = t;
-        }
-        vsync();
       }
     }
     // the next queue (global memory) is read back below: its stores complete
