@@ -16,17 +16,22 @@
 // per launch ("round").  Between rounds each walker is a 12-byte record
 // {vertex, secret, walker id} in the bucket of the partition holding its
 // vertex; partitions are contiguous vertex ranges whose rows and adjacency
-// (~0.7 MB) fit an XCD's L2.  Workgroup b serves the partitions p with
-// p % 8 == b % 8 in increasing order -- workgroups are dealt round-robin to
-// the eight XCDs, so each XCD's L2 holds the partitions its workgroups are
-// on -- and every gather of a round is an L2 hit, while the records stream
-// through HBM once per hop in coalesced runs.
+// (~0.7 MB) fit an XCD's L2.  Each partition is dealt to one XCD (heaviest
+// first by degree sum, the rate at which mixed walks visit it) and workgroup
+// b serves, in increasing order, the partitions dealt to XCD b % 8 --
+// workgroups are dealt round-robin to the eight XCDs, so each XCD's L2 holds
+// the partitions its workgroups are on -- and every gather of a round is an
+// L2 hit, while the records stream through HBM once per hop in coalesced
+// runs.
 //
-// Records of one round are written in blocks of kBR records, taken from a
-// pool with one atomic per block; a workgroup sorts each tile of up to kBR
-// outgoing records by destination partition in LDS and appends each run to
-// its current block of that partition.  A small kernel then lists every
-// partition's blocks for the next round.
+// Records of one round are written in blocks of kBR records, each workgroup
+// taking blocks in order from a region of the pool sized to its input (no
+// atomics); a workgroup sorts each tile of up to kBR outgoing records by
+// destination partition in LDS and appends each run to its current block of
+// that partition.  A small kernel then lists every partition's blocks for
+// the next round.  The pools of a pass take up to 96 GiB: fewer, larger
+// source chunks, and each round's fixed share (a partial block per
+// partition and workgroup, the launch's tail) paid fewer times.
 //
 // Visits are counted when a record is read (the walker has arrived): in LDS
 // counters of the partition, flushed per (workgroup, partition) by plain
