@@ -207,7 +207,7 @@ struct SrcArgs {
   const int32_t* taus;
   float* preds;
   uint64_t* nr_pos;
-  uint32_t* qbuf;     // [slots][2n]
+  uint32_t* qbuf;     // [slots][2n + 128]: two queues of n entries, 64 spare words after each
   float* pvec_g;      // [slots][n]   (global-memory variant)
   uint8_t* seen_g;    // [slots][n]   (global-memory variant)
   uint32_t* next_src;
@@ -309,13 +309,15 @@ __device__ void faithful_source(const SrcArgs& A, uint32_t src, float* pvec, uin
     auto nb_deg = [&](uint2 r, uint32_t u) { return A.degf[ln < r.y ? u : 0u]; };
     // one chunk of 64 row entries of frontier vertex v: u is this lane's
     // entry (0xFFFFFFFF past the row), g its degree as a float
-    auto push_chunk = [&](const uint2 rv, uint32_t base, uint32_t u, float residue, float g) {
+    // `before` is the row entry preceding the chunk (base > 0)
+    auto push_chunk = [&](const uint2 rv, uint32_t base, uint32_t u, float residue, float g,
+                          uint32_t before) {
       const uint32_t j = base + ln;
       const bool valid = j < rv.y;
       const uint32_t up = __shfl_up(u, 1);
       const bool lead = valid && (ln == 0 || u != up);
       bool newrun = lead;
-      if (ln == 0 && base > 0 && valid) newrun = (u != A.adj[rv.x + base - 1]);
+      if (ln == 0 && base > 0 && valid) newrun = (u != before);
       const uint64_t leads = __ballot(lead);
       const uint32_t nvalid = min(64u, rv.y - base);
       uint32_t run = 0;
@@ -326,10 +328,10 @@ __device__ void faithful_source(const SrcArgs& A, uint32_t src, float* pvec, uin
       }
       const bool fresh = newrun && !seen[u];
       const uint64_t fm = __ballot(fresh);
-      if (fresh) {
-        qn[qn_size + __popcll(fm & lanes_below())] = u;
-        seen[u] = 1;
-      }
+      // every lane stores (the others to the queue's spare words past n),
+      // so the count of memory operations does not depend on the data
+      qn[fresh ? qn_size + __popcll(fm & lanes_below()) : n + ln] = u;
+      if (fresh) seen[u] = 1;
       qn_size += __popcll(fm);
       if (lead) {
         const float c = residue / g;
@@ -377,11 +379,24 @@ __device__ void faithful_source(const SrcArgs& A, uint32_t src, float* pvec, uin
           vsync();
           if (ln == 0) pvec[v] = 0.0f;
           vsync();
-          if (rv.y != 0) push_chunk(rv, 0u, u_first, residue, g_first);
-          for (uint32_t base = 64; base < rv.y; base += 64) {
-            const uint32_t j = base + ln;
-            const uint32_t u = j < rv.y ? A.adj[rv.x + j] : 0xFFFFFFFFu;
-            push_chunk(rv, base, u, residue, j < rv.y ? A.degf[u] : 1.0f);
+          if (rv.y != 0) push_chunk(rv, 0u, u_first, residue, g_first, 0u);
+          if (rv.y > 64u) {  // uniform
+            // later chunks: each chunk's ids load during the chunk before
+            // (branch-free: entries past the row read entry 0, dropped)
+            auto ids_at = [&](uint32_t base) {
+              return A.adj[base + ln < rv.y ? rv.x + base + ln : 0u];
+            };
+            uint32_t before = __shfl(u_first, 63);  // the first chunk is full
+            uint32_t uc = ids_at(64u);
+            for (uint32_t base = 64; base < rv.y; base += 64) {
+              const bool in = base + ln < rv.y;
+              const float g = A.degf[in ? uc : 0u];
+              const uint32_t un = ids_at(base + 64u);
+              const uint32_t u = in ? uc : 0xFFFFFFFFu;
+              push_chunk(rv, base, u, residue, g, before);
+              before = __shfl(uc, 63);
+              uc = un;
+            }
           }
         }
       }
@@ -679,8 +694,9 @@ __global__ void __launch_bounds__(64) source_faithful_kernel(SrcArgs A) {
     pvec = A.pvec_g + static_cast<uint64_t>(blockIdx.x) * n;
     seen = A.seen_g + static_cast<uint64_t>(blockIdx.x) * n;
   }
-  uint32_t* qa = A.qbuf + static_cast<uint64_t>(blockIdx.x) * 2ull * n;
-  uint32_t* qb = qa + n;
+  // two queues of n entries, each followed by 64 spare words
+  uint32_t* qa = A.qbuf + static_cast<uint64_t>(blockIdx.x) * (2ull * n + 128u);
+  uint32_t* qb = qa + n + 64u;
   for (uint32_t i = lane_id(); i < n; i += 64) seen[i] = 0;
   __syncthreads();
   const uint32_t count = A.src_list ? A.n_src : n;
@@ -953,7 +969,8 @@ static int run_score_faithful(tgtp_ctx* ctx, tgtp_dgraph* dg, const tgtp_eigens*
   const uint64_t n_run = sl ? std::max<uint32_t>(sl->count, 1u) : n;
   const uint32_t slots =
       static_cast<uint32_t>(std::min<uint64_t>(n_run, static_cast<uint64_t>(ctx->num_cus) * per_cu));
-  auto* d_q = static_cast<uint32_t*>(ctx_buffer(ctx, kBufQueue, static_cast<uint64_t>(slots) * 2 * n * 4));
+  auto* d_q = static_cast<uint32_t*>(
+      ctx_buffer(ctx, kBufQueue, static_cast<uint64_t>(slots) * (2 * n + 128) * 4));
   auto* d_words = static_cast<uint32_t*>(ctx_buffer(ctx, kBufSums, 64));
   float* d_pv = nullptr;
   if (!use_lds) {
