@@ -786,13 +786,16 @@ __device__ __forceinline__ void hop_independent(const FixParams& F, uint32_t& cu
 }
 
 // Replays every marked bouquet, one 8-lane group per bouquet, a lane per
-// walker: t* = first hop at whose start a live lane holds sec == 0 (a
-// minimum over the group); from t* on each lane's independent visits are
-// subtracted and its SplitMix32 visits added.  A wave scans 64 bitmap words at
-// a time and hands their marked bouquets to its eight groups.
+// walker, the group in lockstep: t* = first hop at whose start a live lane
+// holds sec == 0 (a lane at a vertex without neighbours stops and leaves the
+// test); from t* on each lane's independent visits are subtracted and its
+// SplitMix32 visits added, the two walks interleaved (two chains of loads in
+// flight).  A wave scans 64 bitmap words at a time and hands their marked
+// bouquets to its eight groups.
 __global__ void __launch_bounds__(256)
 part_fixup_kernel(FixParams F) {
   const uint32_t lane = threadIdx.x & 63u, grp = lane >> 3, b = lane & 7u;
+  const uint64_t gmask = 0xFFull << (grp * 8u);
   const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6;
   const uint32_t n_waves = (gridDim.x * 256u) >> 6;
   for (uint32_t base = wave * 64u; base < F.n_words; base += n_waves * 64u) {
@@ -814,48 +817,33 @@ part_fixup_kernel(FixParams F) {
         const uint32_t sl = wid0 / F.W;
         const uint32_t init = (wid0 - sl * F.W + b + 1u) * F.step;
         const uint32_t src = F.src_first + sl * F.src_stride;
-        const bool go = has && F.rowd[src].y != 0u;
-        uint32_t tb = F.depth;  // this lane's first hop with a zero secret while live
-        if (go) {
-          uint32_t cur = src, sec = init;
-          for (uint32_t t = 0; t < F.depth; ++t) {
-            const uint2 rd = F.rowd[cur];
-            if (rd.y == 0u) break;
-            if (sec == 0u) {
-              tb = t;
-              break;
-            }
-            hop_independent(F, cur, sec, rd);
+        // lockstep over the group's hops until one starts with a zero secret
+        uint32_t cur = src, sec = init;
+        bool live = has;
+        uint32_t ts = F.depth;
+        for (uint32_t t = 0; t < F.depth; ++t) {
+          const uint2 rd = live ? F.rowd[cur] : make_uint2(0u, 0u);
+          const bool now = live && rd.y != 0u;
+          // (each group's lanes take the same branch: the ballot is per group)
+          if ((__ballot(now && sec == 0u) & gmask) != 0ull) {
+            ts = t;
+            break;
           }
+          live = now;
+          if (live) hop_independent(F, cur, sec, rd);
         }
-        uint32_t ts = tb;
-        for (int o = 1; o < 8; o <<= 1) ts = min(ts, static_cast<uint32_t>(__shfl_xor(ts, o)));
-        if (go && ts < F.depth) {
-          uint32_t cur = src, sec = init;
-          bool live = true;
-          for (uint32_t t = 0; t < ts; ++t) {
-            const uint2 rd = F.rowd[cur];
-            if (rd.y == 0u) {
-              live = false;
-              break;
-            }
-            hop_independent(F, cur, sec, rd);
-          }
-          if (live) {
-            uint32_t c1 = cur, s1 = sec;
-            for (uint32_t t = ts; t < F.depth; ++t) {
-              const uint2 rd = F.rowd[c1];
-              if (rd.y == 0u) break;
-              hop_independent(F, c1, s1, rd);
-              atomicAdd(&F.counters[c1], 0xFFFFFFFFu);
-            }
-            uint32_t c2 = cur, st = init;
-            for (uint32_t t = ts; t < F.depth; ++t) {
-              const uint2 rd = F.rowd[c2];
-              if (rd.y == 0u) break;
-              c2 = F.adj[rd.x + splitmix32(st) % rd.y];
-              atomicAdd(&F.counters[c2], 1u);
-            }
+        if (ts < F.depth && live) {
+          uint32_t c1 = cur, s1 = sec, c2 = cur, st = init;
+          bool a1 = true, a2 = true;
+          for (uint32_t t = ts; t < F.depth && (a1 || a2); ++t) {
+            const uint2 r1 = a1 ? F.rowd[c1] : make_uint2(0u, 0u);
+            const uint2 r2 = a2 ? F.rowd[c2] : make_uint2(0u, 0u);
+            a1 = a1 && r1.y != 0u;
+            a2 = a2 && r2.y != 0u;
+            if (a1) hop_independent(F, c1, s1, r1);
+            if (a2) c2 = F.adj[r2.x + splitmix32(st) % r2.y];
+            if (a1) atomicAdd(&F.counters[c1], 0xFFFFFFFFu);
+            if (a2) atomicAdd(&F.counters[c2], 1u);
           }
         }
       }
