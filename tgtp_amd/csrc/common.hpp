@@ -148,6 +148,9 @@ int launch_mt_stream(tgtp_ctx* ctx, uint32_t seed, uint64_t count, uint32_t* d_o
 // Partitioned walk engine (saba_part.hip) for large graphs: plans the
 // partitions (cached on the graph) and reports whether a pass of these
 // parameters takes it; then runs a pass, adding into cleared counters.
+// Packed walk layout walk_adj = {id | row start | degree} (saba_walk.hip), built
+// once per graph when the fields fit 64 bits; *packed reports whether it did.
+int ensure_walk_adj(tgtp_ctx* ctx, tgtp_dgraph* dg, bool* packed);
 int saba_part_prepare(tgtp_ctx* ctx, tgtp_dgraph* dg, uint64_t n_src, uint32_t depth,
                       uint32_t n_walks, bool* use);
 int launch_saba_walks_part(tgtp_ctx* ctx, tgtp_dgraph* dg, uint32_t src_begin, uint64_t n_src,

@@ -768,6 +768,8 @@ part_reduce_kernel(const uint32_t* __restrict__ copies, uint32_t n_copies, uint6
 struct FixParams {
   const uint2* rowd;
   const uint32_t* adj;
+  const uint64_t* wadj;  // packed {id | row start | degree} layout, or null
+  uint32_t ib, sb;       // its id and row-start field widths
   const uint32_t* taint;
   uint32_t* counters;
   uint32_t n_words;
@@ -778,11 +780,29 @@ struct FixParams {
   uint32_t src_stride;
 };
 
-__device__ __forceinline__ void hop_independent(const FixParams& F, uint32_t& cur, uint32_t& sec,
-                                                uint2 rd) {
-  const uint32_t k = __umulhi(sec ^ fmix32(cur), rd.y);
-  sec *= rd.y;
-  cur = F.adj[rd.x + k];
+// A walker's position: its vertex and that vertex's row (start, degree).
+struct Pos {
+  uint32_t v, start, deg;
+};
+
+template <bool kPacked>
+__device__ __forceinline__ Pos pos_at(const FixParams& F, uint32_t v) {
+  const uint2 rd = F.rowd[v];
+  return Pos{v, rd.x, rd.y};
+}
+
+// The position after taking row entry k: one load with the packed layout,
+// the neighbour and then its row otherwise.
+template <bool kPacked>
+__device__ __forceinline__ Pos step_to(const FixParams& F, const Pos& p, uint32_t k) {
+  if constexpr (kPacked) {
+    const uint64_t e = F.wadj[p.start + k];
+    return Pos{static_cast<uint32_t>(e & ((1ull << F.ib) - 1ull)),
+               static_cast<uint32_t>((e >> F.ib) & ((1ull << F.sb) - 1ull)),
+               static_cast<uint32_t>(e >> (F.ib + F.sb))};
+  } else {
+    return pos_at<false>(F, F.adj[p.start + k]);
+  }
 }
 
 // Replays every marked bouquet, one 8-lane group per bouquet, a lane per
@@ -792,6 +812,7 @@ __device__ __forceinline__ void hop_independent(const FixParams& F, uint32_t& cu
 // SplitMix32 visits added, the two walks interleaved (two chains of loads in
 // flight).  A wave scans 64 bitmap words at a time and hands their marked
 // bouquets to its eight groups.
+template <bool kPacked>
 __global__ void __launch_bounds__(256)
 part_fixup_kernel(FixParams F) {
   const uint32_t lane = threadIdx.x & 63u, grp = lane >> 3, b = lane & 7u;
@@ -818,32 +839,39 @@ part_fixup_kernel(FixParams F) {
         const uint32_t init = (wid0 - sl * F.W + b + 1u) * F.step;
         const uint32_t src = F.src_first + sl * F.src_stride;
         // lockstep over the group's hops until one starts with a zero secret
-        uint32_t cur = src, sec = init;
+        Pos p = has ? pos_at<kPacked>(F, src) : Pos{src, 0u, 0u};
+        uint32_t sec = init;
         bool live = has;
         uint32_t ts = F.depth;
         for (uint32_t t = 0; t < F.depth; ++t) {
-          const uint2 rd = live ? F.rowd[cur] : make_uint2(0u, 0u);
-          const bool now = live && rd.y != 0u;
+          const bool now = live && p.deg != 0u;
           // (each group's lanes take the same branch: the ballot is per group)
           if ((__ballot(now && sec == 0u) & gmask) != 0ull) {
             ts = t;
             break;
           }
           live = now;
-          if (live) hop_independent(F, cur, sec, rd);
+          if (live) {
+            const uint32_t k = __umulhi(sec ^ fmix32(p.v), p.deg);
+            sec *= p.deg;
+            p = step_to<kPacked>(F, p, k);
+          }
         }
         if (ts < F.depth && live) {
-          uint32_t c1 = cur, s1 = sec, c2 = cur, st = init;
+          Pos p1 = p, p2 = p;
+          uint32_t s1 = sec, st = init;
           bool a1 = true, a2 = true;
           for (uint32_t t = ts; t < F.depth && (a1 || a2); ++t) {
-            const uint2 r1 = a1 ? F.rowd[c1] : make_uint2(0u, 0u);
-            const uint2 r2 = a2 ? F.rowd[c2] : make_uint2(0u, 0u);
-            a1 = a1 && r1.y != 0u;
-            a2 = a2 && r2.y != 0u;
-            if (a1) hop_independent(F, c1, s1, r1);
-            if (a2) c2 = F.adj[r2.x + splitmix32(st) % r2.y];
-            if (a1) atomicAdd(&F.counters[c1], 0xFFFFFFFFu);
-            if (a2) atomicAdd(&F.counters[c2], 1u);
+            a1 = a1 && p1.deg != 0u;
+            a2 = a2 && p2.deg != 0u;
+            if (a1) {
+              const uint32_t k = __umulhi(s1 ^ fmix32(p1.v), p1.deg);
+              s1 *= p1.deg;
+              p1 = step_to<kPacked>(F, p1, k);
+            }
+            if (a2) p2 = step_to<kPacked>(F, p2, splitmix32(st) % p2.deg);
+            if (a1) atomicAdd(&F.counters[p1.v], 0xFFFFFFFFu);
+            if (a2) atomicAdd(&F.counters[p2.v], 1u);
           }
         }
       }
@@ -1152,6 +1180,16 @@ int launch_saba_walks_part(tgtp_ctx* ctx, tgtp_dgraph* dg, uint32_t src_begin, u
   FixParams F{};
   F.rowd = dg->rowd;
   F.adj = dg->adj;
+  {
+    // the replay's hops read the packed layout when it can be built (one
+    // load per hop instead of two)
+    bool packed = false;
+    int prc = ensure_walk_adj(ctx, dg, &packed);
+    if (prc) return prc;
+    F.wadj = packed ? dg->walk_adj : nullptr;
+    F.ib = dg->id_bits;
+    F.sb = dg->start_bits;
+  }
   F.taint = taint;
   F.counters = d_counters;
   F.W = n_walks;
@@ -1201,8 +1239,11 @@ int launch_saba_walks_part(tgtp_ctx* ctx, tgtp_dgraph* dg, uint32_t src_begin, u
     }
     F.src_first = A.src_first;
     F.n_words = static_cast<uint32_t>(words);
-    part_fixup_kernel<<<static_cast<uint32_t>(std::min<uint64_t>(2048, (words + 255) / 256)), 256,
-                        0, s>>>(F);
+    const uint32_t fix_grid = static_cast<uint32_t>(std::min<uint64_t>(2048, (words + 255) / 256));
+    if (F.wadj)
+      part_fixup_kernel<true><<<fix_grid, 256, 0, s>>>(F);
+    else
+      part_fixup_kernel<false><<<fix_grid, 256, 0, s>>>(F);
     TGTP_HIP_CHECK(ctx, hipGetLastError());
   }
   part_reduce_kernel<<<static_cast<uint32_t>(std::min<uint64_t>(4096, (n + 255) / 256)), 256, 0,

@@ -368,6 +368,8 @@ uint32_t bits_for(uint64_t x) {  // bits needed to hold values 0..x
   return b;
 }
 
+}  // namespace
+
 // Builds the packed walk layout once per graph when its three fields fit in
 // 64 bits; *packed reports whether the walk can use it.
 int ensure_walk_adj(tgtp_ctx* ctx, tgtp_dgraph* dg, bool* packed) {
@@ -390,6 +392,8 @@ int ensure_walk_adj(tgtp_ctx* ctx, tgtp_dgraph* dg, bool* packed) {
   *packed = true;
   return TGTP_OK;
 }
+
+namespace {
 
 const void* walk_kernel_ptr(bool packed, bool bucket) {
   if (packed)
