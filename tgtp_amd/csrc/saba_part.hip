@@ -984,18 +984,26 @@ bool part_shape(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint64_t n_src, uint32_t n
   // memory is not queried once they hold the cap).
   const uint64_t pool_cap_bytes =
       static_cast<uint64_t>(std::max(64, env_int("TGTP_WALK_PART_MIB", 98304))) << 20;
-  const uint64_t have = ctx->bufs[kBufPartRecA].bytes + ctx->bufs[kBufPartRecB].bytes;
-  uint64_t budget = pool_cap_bytes;
-  if (have < pool_cap_bytes) {
-    size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return false;
-    budget = std::min<uint64_t>(pool_cap_bytes, std::max<uint64_t>(have, (free_b + have) / 4 * 3));
-  }
   // a round's regions hold its input blocks plus P per workgroup, and its
   // input holds at most the walkers' full blocks plus P partial blocks per
   // workgroup of the round before
   const uint64_t slack_blocks = 2ull * S->grid * S->P;
   const uint64_t blk_bytes = static_cast<uint64_t>(kBR) * 12u + 4u * 4u;  // records + tables
+  const uint64_t have = ctx->bufs[kBufPartRecA].bytes + ctx->bufs[kBufPartRecB].bytes;
+  // blocks of a pool holding the whole pass in one chunk, and the bytes of
+  // both such pools
+  const uint64_t blocks_all = (n_src * n_walks + kBR - 1) / kBR + slack_blocks;
+  uint64_t budget = pool_cap_bytes;
+  if (have >= pool_cap_bytes) {
+    budget = pool_cap_bytes;
+  } else if (n_src * n_walks <= 0xFFFFFFF8ull && blocks_all + 1 <= kMaxPoolBlocks &&
+             have >= 2ull * (blocks_all + 1) * kBR * 12u) {
+    budget = 2ull * (blocks_all + 1) * blk_bytes;  // the pools already hold it: one chunk
+  } else {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return false;
+    budget = std::min<uint64_t>(pool_cap_bytes, std::max<uint64_t>(have, (free_b + have) / 4 * 3));
+  }
   const uint64_t pool_blocks_max = std::min<uint64_t>(kMaxPoolBlocks, budget / 2 / blk_bytes);
   if (pool_blocks_max <= slack_blocks + 1) return false;
   const uint64_t walkers_max =
