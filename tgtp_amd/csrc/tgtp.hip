@@ -289,19 +289,16 @@ __device__ void faithful_source(const SrcArgs& A, uint32_t src, float* pvec, uin
     // of the current one, the vertex id, its row descriptor, this lane's
     // neighbour in the row's first 64 entries and that neighbour's degree,
     // each stage kPipeGap entries after the one it depends on, so a step waits
-    // only for loads issued kPipeGap steps earlier.  Entries past the frontier
-    // repeat the source (a valid vertex).  Entry e's stages live in slot
-    // e % kSlots and the loop is unrolled kSlots times, so that no register
-    // holding a load in flight is copied (a copy waits for the load), and the
-    // loads are branch-free (an out-of-range slot reads a valid word and
-    // drops it): the compiler's waits then count only the loads a step uses.
-    // The loaded words are used as they come (no select or copy touches a
-    // load in flight): an entry past the frontier reads q[0]; a lane past
-    // its row reads adj[0] and its degree degf[0], values the push ignores.
-    // (the queue index carries a zero the compiler cannot see as uniform, so
-    // the vertex ids and row descriptors in flight stay in vector registers
-    // until their step instead of being moved to scalar ones at once, which
-    // waits for the load)
+    // only for loads issued kPipeGap steps earlier.  Entry e's stages live in
+    // slot e % kSlots and the loop is unrolled kSlots times, so that no
+    // register holding a load in flight is copied, selected or moved to a
+    // scalar register before its step (each of those waits for the load).
+    // The loads are branch-free and their words are used as they come: an
+    // entry past the frontier reads q[0] (the frontier is never empty: every
+    // vertex has a neighbour), a lane past its row reads adj[0] and degf[0],
+    // values the push ignores.  The queue index carries a zero the compiler
+    // cannot see as uniform, which keeps the ids and row descriptors in
+    // vector registers.
     uint32_t zv;
     __asm__ volatile("v_mov_b32 %0, 0" : "=v"(zv));
     auto qv = [&](uint32_t k) { return q[(k < qsize ? k : 0u) + zv]; };
@@ -354,7 +351,7 @@ __device__ void faithful_source(const SrcArgs& A, uint32_t src, float* pvec, uin
       Gs[e] = 1.0f;
     }
 #pragma unroll
-    for (uint32_t e = 0; e < 3u * kPipeGap; ++e) Rs[e] = A.rowd[Vs[e]];
+    for (uint32_t e = 0; e < 3u * kPipeGap; ++e) Rs[e] = A.rowd[min(Vs[e], n - 1u)];
 #pragma unroll
     for (uint32_t e = 0; e < 2u * kPipeGap; ++e) Us[e] = first_nb(Rs[e]);
 #pragma unroll
@@ -373,7 +370,7 @@ __device__ void faithful_source(const SrcArgs& A, uint32_t src, float* pvec, uin
           constexpr uint32_t kM = kSlots - 1u;
           Gs[(e + kPipeGap) & kM] = nb_deg(Rs[(e + kPipeGap) & kM], Us[(e + kPipeGap) & kM]);
           Us[(e + 2u * kPipeGap) & kM] = first_nb(Rs[(e + 2u * kPipeGap) & kM]);
-          Rs[(e + 3u * kPipeGap) & kM] = A.rowd[Vs[(e + 3u * kPipeGap) & kM]];
+          Rs[(e + 3u * kPipeGap) & kM] = A.rowd[min(Vs[(e + 3u * kPipeGap) & kM], n - 1u)];
           Vs[e] = qv(i + kSlots);
           const float residue = pvec[v];
           vsync();
