@@ -785,7 +785,6 @@ struct Pos {
   uint32_t v, start, deg;
 };
 
-template <bool kPacked>
 __device__ __forceinline__ Pos pos_at(const FixParams& F, uint32_t v) {
   const uint2 rd = F.rowd[v];
   return Pos{v, rd.x, rd.y};
@@ -801,7 +800,7 @@ __device__ __forceinline__ Pos step_to(const FixParams& F, const Pos& p, uint32_
                static_cast<uint32_t>((e >> F.ib) & ((1ull << F.sb) - 1ull)),
                static_cast<uint32_t>(e >> (F.ib + F.sb))};
   } else {
-    return pos_at<false>(F, F.adj[p.start + k]);
+    return pos_at(F, F.adj[p.start + k]);
   }
 }
 
@@ -839,7 +838,7 @@ part_fixup_kernel(FixParams F) {
         const uint32_t init = (wid0 - sl * F.W + b + 1u) * F.step;
         const uint32_t src = F.src_first + sl * F.src_stride;
         // lockstep over the group's hops until one starts with a zero secret
-        Pos p = has ? pos_at<kPacked>(F, src) : Pos{src, 0u, 0u};
+        Pos p = has ? pos_at(F, src) : Pos{src, 0u, 0u};
         uint32_t sec = init;
         bool live = has;
         uint32_t ts = F.depth;
