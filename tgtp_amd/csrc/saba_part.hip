@@ -275,11 +275,16 @@ __device__ __forceinline__ void copy_out(PartLds& L, const PartRound& A) {
 // next tile's record loads are issued at the top of the tile, before the
 // gathers; issued after them -- the next tile's or the one after's -- the
 // pass was 8-10% slower.)
-template <bool kLdsRows>
+//
+// `arrive` (a middle round) counts the tile's arrivals in the partition's LDS
+// counters once the gathers are issued: LDS operations complete in issue
+// order, so counting first would hold up the row offsets the gathers need.
+template <bool kLdsRows, typename Arrive>
 __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A, uint32_t hop,
                                                  const uint32_t* ridx, uint32_t lo,
                                                  uint32_t (&cur)[kPer], uint32_t (&sec)[kPer],
-                                                 uint32_t (&wid)[kPer], bool& pending) {
+                                                 uint32_t (&wid)[kPer], bool& pending,
+                                                 Arrive&& arrive) {
   const uint32_t tid = threadIdx.x;
   bool valid[kPer];  // kNone marks an empty slot
 #pragma unroll
@@ -311,6 +316,7 @@ __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A,
   }
   // keeps the gathers ahead of the stores in issue order
   __asm__ volatile("" ::: "memory");
+  arrive();
   copy_out(L, A);  // the previous tile (none yet: L.total is 0)
   if (A.tadj) {  // uniform
 #pragma unroll
@@ -517,7 +523,7 @@ part_round_kernel(PartRound A) {
         }
 #pragma unroll
         for (uint32_t j = 0; j < kPer; ++j) cur[j] = valid[j] ? cur[j] : kNone;
-        step_and_scatter<false>(L, A, 1u, nullptr, 0u, cur, sec, wid, pending);
+        step_and_scatter<false>(L, A, 1u, nullptr, 0u, cur, sec, wid, pending, [] {});
         // the barrier at the end of step_and_scatter follows every count
         const uint32_t ns0 = r0.y, nsl = r0.y + (sl1 != sl0 ? r1.y : 0u);
         for (uint32_t i = tid; i < nsl; i += kPBlock) {
@@ -554,7 +560,8 @@ part_round_kernel(PartRound A) {
         }
 #pragma unroll
         for (uint32_t j = 0; j < kPer; ++j) cur[j] = valid[j] ? cur[j] : kNone;
-        step_and_scatter<false>(L, A, A.gen_hops - 1u, nullptr, 0u, cur, sec, wid, pending);
+        step_and_scatter<false>(L, A, A.gen_hops - 1u, nullptr, 0u, cur, sec, wid, pending,
+                                [] {});
         // the barrier at the end of step_and_scatter follows every insert
         const uint32_t nh = L.hn;
         for (uint32_t i = tid; i < nh; i += kPBlock) {
@@ -569,7 +576,7 @@ part_round_kernel(PartRound A) {
       } else {
 #pragma unroll
         for (uint32_t j = 0; j < kPer; ++j) cur[j] = valid[j] ? cur[j] : kNone;
-        step_and_scatter<false>(L, A, 0u, nullptr, 0u, cur, sec, wid, pending);
+        step_and_scatter<false>(L, A, 0u, nullptr, 0u, cur, sec, wid, pending, [] {});
       }
     }
   } else {
@@ -624,16 +631,20 @@ part_round_kernel(PartRound A) {
             lds_barrier();
             since_flush = 1;
           }
+          auto arrive = [&] {
 #pragma unroll
-          for (uint32_t j = 0; j < kPer; ++j) {
-            if (cur[j] != kNone) {
-              const uint32_t o = cur[j] - lo;
-              if constexpr (kCnt16) atomicAdd(&cnt[o >> 1], 1u << ((o & 1u) << 4));
-              else atomicAdd(&cnt[o], 1u);
+            for (uint32_t j = 0; j < kPer; ++j) {
+              if (cur[j] != kNone) {
+                const uint32_t o = cur[j] - lo;
+                if constexpr (kCnt16) atomicAdd(&cnt[o >> 1], 1u << ((o & 1u) << 4));
+                else atomicAdd(&cnt[o], 1u);
+              }
             }
-          }
+          };
           if constexpr (kMode == kMid)
-            step_and_scatter<true>(L, A, A.round, ridx, lo, cur, sec, wid, pending);
+            step_and_scatter<true>(L, A, A.round, ridx, lo, cur, sec, wid, pending, arrive);
+          else
+            arrive();
 #pragma unroll
           for (uint32_t j = 0; j < kPer; ++j) {
             cur[j] = ncur[j];
