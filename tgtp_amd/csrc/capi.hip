@@ -219,10 +219,11 @@ static int mt_walks_check(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_beg
 static int mt_walks_run(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin, uint32_t src_end,
                         uint32_t depth, uint32_t n_walks, int32_t variant, uint32_t* counters_out,
                         uint64_t* visit_count, uint64_t* checksum, tgtp_stats* stats,
-                        uint32_t* d_counters, unsigned long long* d_sums, double t0) {
+                        uint32_t* d_counters, unsigned long long* d_sums, double t0,
+                        bool may_alloc) {
   float walk_ms = 0;
   int rc = launch_mt_walks(ctx, dg, src_begin, src_end, depth, n_walks, variant, d_counters,
-                           d_sums, &walk_ms);
+                           d_sums, &walk_ms, may_alloc);
   if (rc) return rc;
   unsigned long long sums[2] = {0, 0};
   TGTP_HIP_CHECK(ctx, hipMemcpyAsync(sums, d_sums, sizeof(sums), hipMemcpyDeviceToHost,
@@ -260,7 +261,7 @@ extern "C" int tgtp_mt_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_
   auto* d_sums = static_cast<unsigned long long*>(ctx_buffer(ctx, kBufSums, 64));
   if (!d_counters || !d_sums) return set_error(ctx, TGTP_E_NOMEM, "counter buffers");
   return mt_walks_run(ctx, dg, src_begin, src_end, depth, n_walks, variant, counters_out,
-                      visit_count, checksum, stats, d_counters, d_sums, t0);
+                      visit_count, checksum, stats, d_counters, d_sums, t0, true);
 }
 
 extern "C" int tgtp_mt_walks_noalloc(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,
@@ -286,7 +287,7 @@ extern "C" int tgtp_mt_walks_noalloc(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint3
   auto* d_counters = static_cast<uint32_t*>(ctx->scratch);
   auto* d_sums = reinterpret_cast<unsigned long long*>(d_counters + kNoAllocMaxN);
   return mt_walks_run(ctx, dg, src_begin, src_end, depth, n_walks, variant, counters_out,
-                      visit_count, checksum, stats, d_counters, d_sums, t0);
+                      visit_count, checksum, stats, d_counters, d_sums, t0, false);
 }
 
 extern "C" int tgtp_saba_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin,

@@ -142,7 +142,7 @@ int launch_walk_counter_sums(tgtp_ctx* ctx, const uint32_t* d_counters, uint64_t
 // MT19937 comparator walks (mt_walk.hip); variant 0 regular, 1 simd.
 int launch_mt_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin, uint32_t src_end,
                     uint32_t depth, uint32_t n_walks, int32_t variant, uint32_t* d_counters,
-                    unsigned long long* d_sums, float* walk_ms);
+                    unsigned long long* d_sums, float* walk_ms, bool may_alloc);
 // The first `count` outputs of std::mt19937(seed), in order (mt_walk.hip).
 int launch_mt_stream(tgtp_ctx* ctx, uint32_t seed, uint64_t count, uint32_t* d_out);
 // Partitioned walk engine (saba_part.hip) for large graphs: plans the

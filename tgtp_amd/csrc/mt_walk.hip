@@ -38,6 +38,8 @@ constexpr uint32_t kMtHot = 4096;   // vertices counted in LDS
 struct MtWalkArgs {
   const uint2* rowd;  // {row start, degree}
   const uint32_t* adj;
+  const uint64_t* wadj;  // packed {id | row start | degree} adjacency, or null
+  uint32_t ib, sb;       // its id and row-start field widths
   uint32_t* counts;
   uint32_t n;
   uint32_t src_begin, src_end;
@@ -95,6 +97,9 @@ __device__ __forceinline__ uint32_t mt_temper(uint32_t y) {
   return y;
 }
 
+// kPacked: a hop reads one packed 8-byte entry (the next vertex, its row
+// start and degree) instead of the neighbour id and then its row.
+template <bool kPacked>
 __global__ void __launch_bounds__(kMtBlock) mt_walk_kernel(MtWalkArgs a) {
   __shared__ uint32_t mt[kMtN];
   __shared__ uint32_t ring[kRing];
@@ -121,17 +126,25 @@ __global__ void __launch_bounds__(kMtBlock) mt_walk_kernel(MtWalkArgs a) {
       if (lane < nw) {
         const uint32_t w = w0 + lane;
         uint32_t cur = s;
+        uint2 d = a.rowd[s];  // {row start, degree} of cur
         for (uint32_t h = 0; h < a.depth; ++h) {
           const uint64_t o = a.variant == 0
                                  ? static_cast<uint64_t>(w) * depth + h
                                  : static_cast<uint64_t>(w / 8u) * 8u * depth + 8u * h + (w % 8u);
           const uint32_t r = ring[o % kRing];
-          const uint2 d = a.rowd[cur];
           // a vertex without neighbours (only in a non-symmetric CSR; the
           // reference divides by zero there): the walker stops, and since
           // each hop's draw index is fixed, no other walk moves in the stream
           if (d.y == 0) break;
-          cur = a.adj[d.x + r % d.y];
+          if constexpr (kPacked) {
+            const uint64_t e = a.wadj[d.x + r % d.y];
+            cur = static_cast<uint32_t>(e & ((1ull << a.ib) - 1ull));
+            d = make_uint2(static_cast<uint32_t>((e >> a.ib) & ((1ull << a.sb) - 1ull)),
+                           static_cast<uint32_t>(e >> (a.ib + a.sb)));
+          } else {
+            cur = a.adj[d.x + r % d.y];
+            d = a.rowd[cur];
+          }
           if (cur < kMtHot)
             atomicAdd(&hot[cur], 1u);
           else
@@ -170,7 +183,7 @@ int launch_mt_stream(tgtp_ctx* ctx, uint32_t seed, uint64_t count, uint32_t* d_o
 
 int launch_mt_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin, uint32_t src_end,
                     uint32_t depth, uint32_t n_walks, int32_t variant, uint32_t* d_counters,
-                    unsigned long long* d_sums, float* walk_ms) {
+                    unsigned long long* d_sums, float* walk_ms, bool may_alloc) {
   hipStream_t s = ctx->stream;
   // a batch's range of the stream plus one twist must fit the ring
   uint32_t wpb = depth ? std::min<uint32_t>(kMtBlock, (kRing - kMtN) / depth) : kMtBlock;
@@ -187,6 +200,13 @@ int launch_mt_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin, ui
     MtWalkArgs a{};
     a.rowd = dg->rowd;
     a.adj = dg->adj;
+    // the packed layout when it can be built here (or is there already: the
+    // entry point that allocates nothing only reads it)
+    bool packed = dg->walk_adj != nullptr;
+    if (may_alloc && (rc = ensure_walk_adj(ctx, const_cast<tgtp_dgraph*>(dg), &packed))) return rc;
+    a.wadj = packed ? dg->walk_adj : nullptr;
+    a.ib = dg->id_bits;
+    a.sb = dg->start_bits;
     a.counts = d_counters;
     a.n = static_cast<uint32_t>(dg->n);
     a.src_begin = src_begin;
@@ -199,11 +219,15 @@ int launch_mt_walks(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint32_t src_begin, ui
     // enough that the per-workgroup flush of the LDS counters stays small
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(&mt_walk_kernel), kMtBlock, 0) != hipSuccess)
+            &per_cu, reinterpret_cast<const void*>(&mt_walk_kernel<true>), kMtBlock, 0) !=
+        hipSuccess)
       per_cu = 1;
     const uint32_t groups = std::min<uint32_t>(
         src_end - src_begin, static_cast<uint32_t>(std::max(1, per_cu)) * ctx->num_cus);
-    mt_walk_kernel<<<groups, kMtBlock, 0, s>>>(a);
+    if (a.wadj)
+      mt_walk_kernel<true><<<groups, kMtBlock, 0, s>>>(a);
+    else
+      mt_walk_kernel<false><<<groups, kMtBlock, 0, s>>>(a);
     TGTP_HIP_CHECK(ctx, hipGetLastError());
   }
   TGTP_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2], s));
