@@ -95,6 +95,16 @@ def test_engine_shapes(gpu_ctx, part, knobs):
     _check(gpu_ctx, G.chung_lu(3000, 2.3, 14.0, seed=5), 9, 64, label=f"part {knobs}")
 
 
+@pytest.mark.parametrize("depth,gen_hops", [(1, "2"), (2, "2"), (3, "2"), (4, "4"), (5, "4")])
+def test_round_feeding_the_last(gpu_ctx, part, depth, gen_hops):
+    """The round whose records the last round reads writes vertices alone:
+    the generation round when it takes every hop (depth 1 and 2 at two
+    generation hops, depth 4 at four), a middle round otherwise."""
+    part.setenv("TGTP_WALK_PART_GEN_HOPS", gen_hops)
+    _check(gpu_ctx, G.chung_lu(3000, 2.3, 14.0, seed=5), depth, 64,
+           label=f"part last-feed d{depth} gen{gen_hops}")
+
+
 @pytest.mark.parametrize("W_,slots", [(6144, "1"), (12288, "1"), (16384, "1"), (16384, "0")])
 def test_generation_first_hop_counting(gpu_ctx, part, W_, slots):
     """Two-hop generation round with walks per source of at least a tile
