@@ -36,7 +36,20 @@ sys.path.insert(0, str(ROOT))
 METRIC = "SABA transitions/s + tgtp edges/s at eps 0.01, batched vs plain, 1 GPU"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_TRANSITION = 20  # row bounds 8 + neighbour 4 + counter RMW 8
-RECORD_BYTES_PER_TRANSITION = 24  # partitioned engine: a 12-byte walker record read and written
+
+
+def record_bytes_per_transition(depth: int, gen_hops: int = 2) -> float:
+    """Walker-record bytes the partitioned engine moves per transition: the
+    generation round (gen_hops hops) writes a 12-byte record, each later hop
+    reads one and writes one, the round that feeds the last round writes the
+    4-byte vertex alone, and the last round reads it."""
+    g = min(gen_hops, depth)
+    mids = depth - g  # hops g .. depth-1, one round each
+    if mids == 0:
+        b = 4 + 4
+    else:
+        b = 12 + 24 * (mids - 1) + (12 + 4) + 4
+    return b / depth
 
 
 T_START = time.perf_counter()
@@ -104,17 +117,18 @@ def bench_saba(args, g=None, depth=None, steps=None, warmup=None):
     log(f"[bench] depth {depth}, {steps} steps: {ms_per_step:.2f} ms/step, walk kernel "
         f"{avg_walk_s * 1e3:.2f} ms, {value:.3e} transitions/s, checksum={cs}")
     # The library walks graphs whose rows and adjacency exceed 64 MiB with the
-    # partitioned engine (saba_part.hip: per source chunk one generation
-    # round, depth-1 middle rounds and a counting round of part_round_kernel,
-    # plus the fallback replay) unless TGTP_WALK_PART=0; smaller graphs with
-    # the direct kernel (saba_walk.hip: saba_walk_kernel + bucket_count_kernel
-    # per chunk).  The unit of "launch" here is the pass, timed over all its
-    # kernels with HIP events on the library's stream.
+    # partitioned engine (saba_part.hip: per source chunk a two-hop
+    # generation round, depth-2 middle rounds and a counting round of
+    # part_round_kernel, plus the fallback replay) unless TGTP_WALK_PART=0;
+    # smaller graphs with the direct kernel (saba_walk.hip: saba_walk_kernel
+    # + bucket_count_kernel per chunk).  The unit of "launch" here is the
+    # pass, timed over all its kernels with HIP events on the library's
+    # stream.
     part = (os.environ.get("TGTP_WALK_PART", "-1") != "0"
             and 4 * g.m + 8 * g.n >= 64 * 2**20)
     if part:
-        kernel = ("part_round_kernel<0,1,2> + part_fixup_kernel + block listing (one pass; "
-                  "the middle rounds part_round_kernel<1> are 84% of it)")
+        kernel = ("part_round_kernel<0|1|2> + part_fixup_kernel + block listing (one pass; "
+                  "the middle rounds part_round_kernel<1> are 84-88% of it)")
         traffic = traffic_for("saba_part_pass", f"cl1m_d{depth}_w{n_walks}")
     else:
         kernel = "saba_walk_kernel + bucket_count_kernel (one pass)"
@@ -751,13 +765,15 @@ def main() -> int:
     copy_gbs = stream_copy_rate()
     if copy_gbs:
         # the partitioned engine's bound: each walker's 12-byte record is read
-        # and written once per hop
-        bound = copy_gbs * 1e9 / RECORD_BYTES_PER_TRANSITION
+        # and written once per round (the last round's input is the vertex alone)
+        rb = record_bytes_per_transition(cfg["depth"])
+        bound = copy_gbs * 1e9 / rb
         roof["record_stream_roof"] = {
             "what": "a 4 GiB device-to-device copy measured in this run (read + write bytes "
-                    "per second) over the partitioned engine's 24 bytes of records per "
-                    "transition (12 B read, 12 B written)",
-            "copy_GBps": round(copy_gbs, 1), "bytes_per_transition": RECORD_BYTES_PER_TRANSITION,
+                    f"per second) over the partitioned engine's {rb:g} bytes of records per "
+                    "transition (12-byte records read and written once per round, the two-hop "
+                    "generation round writing one, the last round reading 4-byte vertices)",
+            "copy_GBps": round(copy_gbs, 1), "bytes_per_transition": rb,
             "transitions_per_s": bound, "frac": round(value / bound, 4)}
         log(f"[bench] record-stream roof {bound:.3e} transitions/s ({copy_gbs:.0f} GB/s copy); "
             f"walk at {value / bound:.2f} of it")
