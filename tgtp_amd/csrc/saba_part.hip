@@ -15,9 +15,10 @@
 // memory system serves bounds it.  Here a chunk of walkers advances one hop
 // per launch ("round").  Between rounds each walker is a 12-byte record
 // {vertex, secret, walker id} in the bucket of the partition holding its
-// vertex (the records the last round reads, which only counts arrivals, are
-// the vertex alone); partitions are contiguous vertex ranges whose rows and
-// adjacency (~0.7 MB) fit an XCD's L2.  Each partition is dealt to one XCD (heaviest
+// vertex (8 bytes {vertex, secret} into the round that takes the last hop,
+// which marks no bouquet, and 4 bytes, the vertex, into the last round, which
+// only counts arrivals); partitions are contiguous vertex ranges whose rows
+// and adjacency (~0.7 MB) fit an XCD's L2.  Each partition is dealt to one XCD (heaviest
 // first by degree sum, the rate at which mixed walks visit it) and workgroup
 // b serves, in increasing order, the partitions dealt to XCD b % 8 --
 // workgroups are dealt round-robin to the eight XCDs, so each XCD's L2 holds
@@ -115,8 +116,8 @@ struct PartRound {
   uint32_t n_walkers;    // walkers of the chunk (kGen)
   uint32_t pool_cap;     // blocks in the output pool
   // input pool (kMid, kLast)
-  const uint32_t* rec_in;     // [blocks][kBR][3] = {vertex, secret, walker id};
-                              // kLast: [blocks][kBR] vertices
+  const uint32_t* rec_in;     // [blocks][kBR][3] = {vertex, secret, walker id},
+                              // [..][2] {vertex, secret} or [..][1] vertex
   const uint32_t* order_in;   // blocks grouped by partition
   const uint32_t* pstart_in;  // [P] first entry of each partition in order_in
   const uint32_t* bcount_in;  // [P] blocks per partition
@@ -256,9 +257,12 @@ __device__ __forceinline__ void count_arrival(PartLds& L, uint32_t* hkey, uint32
 // counts exactly the loads and stores issued after the gathers instead of
 // waiting for every outstanding access.
 //
-// kVOut: the records feed the last round, which only counts arrivals, so a
-// record is its vertex alone (4 bytes at word g instead of 12 at 3 g).
-template <bool kVOut>
+// kOutW: 32-bit words per record written.  3: {vertex, secret, walker id}.
+// 2: {vertex, secret}, for the round that takes the last hop (a bouquet is
+// marked only while more hops follow, so it needs no walker id).  1: the
+// vertex alone, for the last round, which only counts arrivals.  Record g
+// is at word kOutW * g.
+template <int kOutW>
 __device__ __forceinline__ void copy_out(PartLds& L, const PartRound& A) {
   const uint32_t total = L.total;
   const uint64_t spare = static_cast<uint64_t>(A.pool_cap) * kBR + threadIdx.x;
@@ -268,14 +272,10 @@ __device__ __forceinline__ void copy_out(PartLds& L, const PartRound& A) {
     const uint4 rn = L.run[L.tdest[q]];
     const uint32_t g = q + (q < rn.x ? rn.y : rn.z);
     const uint64_t e = q < total ? static_cast<uint64_t>(g) : spare;
-    if constexpr (kVOut) {
-      __builtin_nontemporal_store(L.tcur[q], A.rec_out + e);
-    } else {
-      uint32_t* r = A.rec_out + e * 3u;
-      __builtin_nontemporal_store(L.tcur[q], r);
-      __builtin_nontemporal_store(L.tsec[q], r + 1);
-      __builtin_nontemporal_store(L.twid[q], r + 2);
-    }
+    uint32_t* r = A.rec_out + e * static_cast<uint32_t>(kOutW);
+    __builtin_nontemporal_store(L.tcur[q], r);
+    if constexpr (kOutW >= 2) __builtin_nontemporal_store(L.tsec[q], r + 1);
+    if constexpr (kOutW >= 3) __builtin_nontemporal_store(L.twid[q], r + 2);
   }
 }
 
@@ -290,7 +290,7 @@ __device__ __forceinline__ void copy_out(PartLds& L, const PartRound& A) {
 // `arrive` (a middle round) counts the tile's arrivals in the partition's LDS
 // counters once the gathers are issued: LDS operations complete in issue
 // order, so counting first would hold up the row offsets the gathers need.
-template <bool kLdsRows, bool kVOut, typename Arrive>
+template <bool kLdsRows, int kOutW, typename Arrive>
 __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A, uint32_t hop,
                                                  const uint32_t* ridx, uint32_t lo,
                                                  uint32_t (&cur)[kPer], uint32_t (&sec)[kPer],
@@ -328,7 +328,7 @@ __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A,
   // keeps the gathers ahead of the stores in issue order
   __asm__ volatile("" ::: "memory");
   arrive();
-  copy_out<kVOut>(L, A);  // the previous tile (none yet: L.total is 0)
+  copy_out<kOutW>(L, A);  // the previous tile (none yet: L.total is 0)
   if (A.tadj) {  // uniform
 #pragma unroll
     for (uint32_t j = 0; j < kPer; ++j) {
@@ -381,8 +381,8 @@ __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A,
     if (valid[j]) {
       const uint32_t pos = L.off[dest[j]] + rank[j];
       L.tcur[pos] = nxt[j];
-      L.tsec[pos] = sec[j];
-      L.twid[pos] = wid[j];
+      if constexpr (kOutW >= 2) L.tsec[pos] = sec[j];
+      if constexpr (kOutW >= 3) L.twid[pos] = wid[j];
       L.tdest[pos] = static_cast<uint8_t>(dest[j]);
     }
   }
@@ -392,9 +392,9 @@ __device__ __forceinline__ void step_and_scatter(PartLds& L, const PartRound& A,
 
 // Loads input record i of block blk (kNone: none) as vertex / secret / id;
 // slots past the block's fill get vertex kNone (branch-free: they read the
-// pool's spare block, and the values are dropped).  kVIn: the records are
-// vertices alone (the last round's input); secret and id read as 0.
-template <bool kVIn>
+// pool's spare block, and the values are dropped).  kInW: words per record
+// (as copy_out's kOutW); fields the records do not hold read as 0.
+template <int kInW>
 __device__ __forceinline__ void load_tile(const PartRound& A, uint32_t blk, uint32_t fill,
                                           uint32_t (&cur)[kPer], uint32_t (&sec)[kPer],
                                           uint32_t (&wid)[kPer]) {
@@ -404,29 +404,31 @@ __device__ __forceinline__ void load_tile(const PartRound& A, uint32_t blk, uint
     const bool ok = blk != kNone && i < fill;
     const uint64_t g = ok ? static_cast<uint64_t>(blk) * kBR + i
                           : static_cast<uint64_t>(A.pool_cap) * kBR + threadIdx.x;
-    if constexpr (kVIn) {
-      const uint32_t c = __builtin_nontemporal_load(A.rec_in + g);
-      cur[j] = ok ? c : kNone;
-      sec[j] = 0u;
-      wid[j] = 0u;
-    } else {
-      const uint32_t* r = A.rec_in + g * 3u;
-      const uint32_t c = __builtin_nontemporal_load(r);
+    const uint32_t* r = A.rec_in + g * static_cast<uint32_t>(kInW);
+    const uint32_t c = __builtin_nontemporal_load(r);
+    cur[j] = ok ? c : kNone;
+    if constexpr (kInW >= 2) {
       const uint32_t x = __builtin_nontemporal_load(r + 1);
-      const uint32_t y = __builtin_nontemporal_load(r + 2);
-      cur[j] = ok ? c : kNone;
       sec[j] = ok ? x : 0u;
+    } else {
+      sec[j] = 0u;
+    }
+    if constexpr (kInW >= 3) {
+      const uint32_t y = __builtin_nontemporal_load(r + 2);
       wid[j] = ok ? y : 0u;
+    } else {
+      wid[j] = 0u;
     }
   }
 }
 
-// kVOut (generation and middle rounds): the output feeds the last round and
-// holds vertices alone.  The last round reads such records.
-template <int kMode, bool kVOut>
+// kInW / kOutW: words per record read (middle and last rounds) and written
+// (generation and middle rounds), as copy_out's kOutW.
+template <int kMode, int kInW, int kOutW>
 __global__ void __launch_bounds__(kPBlock)
 part_round_kernel(PartRound A) {
-  static_assert(kMode != kLast || !kVOut, "the last round writes no records");
+  static_assert((kMode == kLast) == (kOutW == 0), "the last round writes no records");
+  static_assert(kMode != kGen || kInW == 0, "the generation round reads no records");
   extern __shared__ __align__(16) uint8_t lds_raw[];
   PartLds& L = *reinterpret_cast<PartLds*>(lds_raw);
   // the current partition's arrival counters and row offsets
@@ -546,7 +548,7 @@ part_round_kernel(PartRound A) {
         }
 #pragma unroll
         for (uint32_t j = 0; j < kPer; ++j) cur[j] = valid[j] ? cur[j] : kNone;
-        step_and_scatter<false, kVOut>(L, A, 1u, nullptr, 0u, cur, sec, wid, pending, [] {});
+        step_and_scatter<false, kOutW>(L, A, 1u, nullptr, 0u, cur, sec, wid, pending, [] {});
         // the barrier at the end of step_and_scatter follows every count
         const uint32_t ns0 = r0.y, nsl = r0.y + (sl1 != sl0 ? r1.y : 0u);
         for (uint32_t i = tid; i < nsl; i += kPBlock) {
@@ -583,7 +585,7 @@ part_round_kernel(PartRound A) {
         }
 #pragma unroll
         for (uint32_t j = 0; j < kPer; ++j) cur[j] = valid[j] ? cur[j] : kNone;
-        step_and_scatter<false, kVOut>(L, A, A.gen_hops - 1u, nullptr, 0u, cur, sec, wid, pending,
+        step_and_scatter<false, kOutW>(L, A, A.gen_hops - 1u, nullptr, 0u, cur, sec, wid, pending,
                                 [] {});
         // the barrier at the end of step_and_scatter follows every insert
         const uint32_t nh = L.hn;
@@ -599,7 +601,7 @@ part_round_kernel(PartRound A) {
       } else {
 #pragma unroll
         for (uint32_t j = 0; j < kPer; ++j) cur[j] = valid[j] ? cur[j] : kNone;
-        step_and_scatter<false, kVOut>(L, A, 0u, nullptr, 0u, cur, sec, wid, pending, [] {});
+        step_and_scatter<false, kOutW>(L, A, 0u, nullptr, 0u, cur, sec, wid, pending, [] {});
       }
     }
   } else {
@@ -642,12 +644,12 @@ part_round_kernel(PartRound A) {
         }
         lds_barrier();
         uint32_t cur[kPer], sec[kPer], wid[kPer];
-        load_tile<kMode == kLast>(A, L.lblk[0], L.lfill[0], cur, sec, wid);
+        load_tile<kInW>(A, L.lblk[0], L.lfill[0], cur, sec, wid);
         for (uint32_t t = 0; t < nl; ++t) {
           // the next tile's records are in flight while this one is stepped
           uint32_t ncur[kPer], nsec[kPer], nwid[kPer];
-          load_tile<kMode == kLast>(A, t + 1 < nl ? L.lblk[t + 1] : kNone,
-                                    t + 1 < nl ? L.lfill[t + 1] : 0u, ncur, nsec, nwid);
+          load_tile<kInW>(A, t + 1 < nl ? L.lblk[t + 1] : kNone,
+                          t + 1 < nl ? L.lfill[t + 1] : 0u, ncur, nsec, nwid);
           if (kCnt16 && ++since_flush > kFlushTiles) {  // uniform
             lds_barrier();
             flush();
@@ -665,7 +667,7 @@ part_round_kernel(PartRound A) {
             }
           };
           if constexpr (kMode == kMid)
-            step_and_scatter<true, kVOut>(L, A, A.round, ridx, lo, cur, sec, wid, pending, arrive);
+            step_and_scatter<true, kOutW>(L, A, A.round, ridx, lo, cur, sec, wid, pending, arrive);
           else
             arrive();
 #pragma unroll
@@ -680,7 +682,8 @@ part_round_kernel(PartRound A) {
       flush();
     }
   }
-  if (pending) copy_out<kVOut>(L, A);
+  if constexpr (kMode != kLast)
+    if (pending) copy_out<kOutW>(L, A);
   if constexpr (kMode != kLast) {
     lds_barrier();
     if (tid < A.P) {
@@ -1002,7 +1005,7 @@ bool part_shape(tgtp_ctx* ctx, const tgtp_dgraph* dg, uint64_t n_src, uint32_t n
            ((kCnt16 ? (S->vbig + 1ull) / 2 : S->vbig) + S->vbig + 1ull) * sizeof(uint32_t);
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, reinterpret_cast<const void*>(&part_round_kernel<kMid, false>), kPBlock, S->lds) !=
+          &per_cu, reinterpret_cast<const void*>(&part_round_kernel<kMid, 3, 3>), kPBlock, S->lds) !=
           hipSuccess ||
       per_cu < 1)
     return false;
@@ -1261,16 +1264,22 @@ int launch_saba_walks_part(tgtp_ctx* ctx, tgtp_dgraph* dg, uint32_t src_begin, u
       A.bfill_out = out.bfill;
       A.bpart_out = out.bpart;
       A.bslot_out = out.bslot;
-      // the round whose records the last round reads writes vertices alone
-      const bool to_last = (r == 0 ? A.gen_hops : r + 1u) == depth;
+      // words per record into the round that takes hop h: 3, but 2 into the
+      // round that takes the last hop (no walker id) and 1 into the last
+      // round (the vertex)
+      auto words_into = [&](uint32_t h) { return h == depth ? 1 : h + 1u == depth ? 2 : 3; };
       if (r == 0) {
-        if (to_last) part_round_kernel<kGen, true><<<grid, kPBlock, lds, s>>>(A);
-        else part_round_kernel<kGen, false><<<grid, kPBlock, lds, s>>>(A);
+        const int ow = words_into(A.gen_hops);
+        if (ow == 1) part_round_kernel<kGen, 0, 1><<<grid, kPBlock, lds, s>>>(A);
+        else if (ow == 2) part_round_kernel<kGen, 0, 2><<<grid, kPBlock, lds, s>>>(A);
+        else part_round_kernel<kGen, 0, 3><<<grid, kPBlock, lds, s>>>(A);
       } else if (r < depth) {
-        if (to_last) part_round_kernel<kMid, true><<<grid, kPBlock, lds, s>>>(A);
-        else part_round_kernel<kMid, false><<<grid, kPBlock, lds, s>>>(A);
+        const int ow = words_into(r + 1u);  // 1: r is the last hop (its input: 2)
+        if (ow == 1) part_round_kernel<kMid, 2, 1><<<grid, kPBlock, lds, s>>>(A);
+        else if (ow == 2) part_round_kernel<kMid, 3, 2><<<grid, kPBlock, lds, s>>>(A);
+        else part_round_kernel<kMid, 3, 3><<<grid, kPBlock, lds, s>>>(A);
       } else {
-        part_round_kernel<kLast, false><<<grid, kPBlock, lds, s>>>(A);
+        part_round_kernel<kLast, 1, 0><<<grid, kPBlock, lds, s>>>(A);
       }
       TGTP_HIP_CHECK(ctx, hipGetLastError());
       if (r < depth) {
