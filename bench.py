@@ -40,16 +40,17 @@ BYTES_PER_TRANSITION = 20  # row bounds 8 + neighbour 4 + counter RMW 8
 
 def record_bytes_per_transition(depth: int, gen_hops: int = 2) -> float:
     """Walker-record bytes the partitioned engine moves per transition: the
-    generation round (gen_hops hops) writes a 12-byte record, each later hop
-    reads one and writes one, the round that feeds the last round writes the
-    4-byte vertex alone, and the last round reads it."""
+    generation round (gen_hops hops) writes the records of the next round,
+    each later round reads its records and writes the next round's, once.
+    A record into the round that takes hop h holds 3 words {vertex, secret,
+    walker id}, 2 {vertex, secret} into the round of the last hop and 1 (the
+    vertex) into the last round, which only counts arrivals."""
     g = min(gen_hops, depth)
-    mids = depth - g  # hops g .. depth-1, one round each
-    if mids == 0:
-        b = 4 + 4
-    else:
-        b = 12 + 24 * (mids - 1) + (12 + 4) + 4
-    return b / depth
+
+    def words_into(h: int) -> int:
+        return 1 if h == depth else 2 if h + 1 == depth else 3
+
+    return sum(2 * 4 * words_into(h) for h in range(g, depth + 1)) / depth
 
 
 T_START = time.perf_counter()
@@ -764,15 +765,16 @@ def main() -> int:
             f"{value / roof_lps:.2f} of it")
     copy_gbs = stream_copy_rate()
     if copy_gbs:
-        # the partitioned engine's bound: each walker's 12-byte record is read
-        # and written once per round (the last round's input is the vertex alone)
+        # the partitioned engine's bound: each walker's record is written and
+        # read once per round (12 bytes; 8 and 4 into the last two rounds)
         rb = record_bytes_per_transition(cfg["depth"])
         bound = copy_gbs * 1e9 / rb
         roof["record_stream_roof"] = {
             "what": "a 4 GiB device-to-device copy measured in this run (read + write bytes "
                     f"per second) over the partitioned engine's {rb:g} bytes of records per "
                     "transition (12-byte records read and written once per round, the two-hop "
-                    "generation round writing one, the last round reading 4-byte vertices)",
+                    "generation round writing one, 8-byte {vertex, secret} records into the "
+                    "round of the last hop, 4-byte vertices into the last round)",
             "copy_GBps": round(copy_gbs, 1), "bytes_per_transition": rb,
             "transitions_per_s": bound, "frac": round(value / bound, 4)}
         log(f"[bench] record-stream roof {bound:.3e} transitions/s ({copy_gbs:.0f} GB/s copy); "
