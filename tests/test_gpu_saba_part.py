@@ -97,9 +97,12 @@ def test_engine_shapes(gpu_ctx, part, knobs):
 
 @pytest.mark.parametrize("depth,gen_hops", [(1, "2"), (2, "2"), (3, "2"), (4, "4"), (5, "4")])
 def test_round_feeding_the_last(gpu_ctx, part, depth, gen_hops):
-    """The round whose records the last round reads writes vertices alone:
-    the generation round when it takes every hop (depth 1 and 2 at two
-    generation hops, depth 4 at four), a middle round otherwise."""
+    """Record widths near the end of a walk: the round whose records the
+    last round reads writes vertices alone, the one before it {vertex,
+    secret}.  The generation round writes the vertices when it takes every
+    hop (depth 1 and 2 at two generation hops, depth 4 at four) and the
+    2-word records when one hop follows it (depth 3 at two, depth 5 at
+    four); middle rounds otherwise."""
     part.setenv("TGTP_WALK_PART_GEN_HOPS", gen_hops)
     _check(gpu_ctx, G.chung_lu(3000, 2.3, 14.0, seed=5), depth, 64,
            label=f"part last-feed d{depth} gen{gen_hops}")
